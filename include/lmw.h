@@ -1,0 +1,223 @@
+/*
+ * lmw.h -- C ABI of the MI355X (gfx950) dynamical core: the drop-in replacement of lmweather's weather_core
+ * library (reference CMakeLists.txt:24: cpp/sphere.cpp + cpp/plane.cpp) below the wx.* operator registrations
+ * (reference cpp/main.cpp:108-495).
+ *
+ * Conventions
+ *   - Every function returns lmw_status.  LMW_EINVAL stands for std::invalid_argument, LMW_ERUNTIME for
+ *     std::runtime_error; lmw_last_error() returns the message (thread-local), byte-identical to the reference's
+ *     exception text.  LMW_EDEVICE reports a HIP failure.  No C++ exception crosses this boundary.
+ *   - All arrays are HOST pointers in the reference's layouts: grids are nlat x nlon row-major, south to north
+ *     (cpp/sphere.hpp:88); spectral arrays are (N+1)^2 complex numbers stored as interleaved (re, im) doubles at
+ *     index m*(N+1)+n (cpp/sphere.hpp:107,110; entries with n < m are zero); plane h and u are ny x nx and v is
+ *     (ny+1) x nx (cpp/plane.hpp:32-37).  The model state stays resident in device memory between calls.
+ *   - Calls are synchronous: results are complete on return.  Handles are not thread-safe (one caller thread
+ *     per handle, like the single-threaded reference).
+ */
+#ifndef LMW_H
+#define LMW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int lmw_status;
+#define LMW_OK 0
+#define LMW_EINVAL 1
+#define LMW_ERUNTIME 2
+#define LMW_EDEVICE 3
+
+/* arithmetic of the Legendre contractions; everything else (FFTs, grid products, RK4, tables, quadrature,
+ * conservation sums) is FP64 in every mode */
+#define LMW_FP64 0
+#define LMW_FP32 1   /* FP32 tables and operands, FP32 accumulation */
+#define LMW_BF16X3 2 /* split BF16 hi+lo operands, three MFMA products, FP32 accumulation */
+
+/* grid fields of lmw_sphere_get_grid (reference grid_height/grid_u/grid_v/grid_vorticity/grid_tracer,
+ * cpp/sphere.hpp:91-95,55) */
+#define LMW_GRID_H 0
+#define LMW_GRID_U 1
+#define LMW_GRID_V 2
+#define LMW_GRID_VORTICITY 3
+#define LMW_GRID_TRACER 4
+
+/* spectral fields (reference zeta_, delta_, phi_, q_: cpp/sphere.hpp:109) */
+#define LMW_SPEC_ZETA 0
+#define LMW_SPEC_DELTA 1
+#define LMW_SPEC_PHI 2
+#define LMW_SPEC_Q 3
+
+/* scalar diagnostics of lmw_sphere_diag (reference cpp/sphere.hpp:70-88,55-62) */
+#define LMW_DIAG_MEAN_HEIGHT 0
+#define LMW_DIAG_ENERGY 1
+#define LMW_DIAG_KINETIC 2
+#define LMW_DIAG_POTENTIAL 3
+#define LMW_DIAG_ENSTROPHY 4
+#define LMW_DIAG_MAX_WIND 5
+#define LMW_DIAG_ERROR_L2 6
+#define LMW_DIAG_ERROR_LINF 7
+#define LMW_DIAG_TIME 8
+
+/* point evaluators of lmw_sphere_point (reference height_at/wind_at/vorticity_at: cpp/sphere.hpp:83-86) */
+#define LMW_POINT_HEIGHT 0
+#define LMW_POINT_U 1
+#define LMW_POINT_V 2
+#define LMW_POINT_VORTICITY 3
+
+/* plane fields and diagnostics (reference cpp/plane.hpp:32-56) */
+#define LMW_PLANE_H 0
+#define LMW_PLANE_U 1
+#define LMW_PLANE_V 2
+#define LMW_PDIAG_MEAN_DEPTH 0
+#define LMW_PDIAG_ENERGY 1
+#define LMW_PDIAG_MAX_SPEED 2
+#define LMW_PDIAG_PV_ANOMALY_RMS 3
+#define LMW_PDIAG_TIME 4
+
+/* reference SphereParams, cpp/sphere.hpp:20-25 */
+typedef struct {
+  double radius;    /* m */
+  double omega;     /* rad/s */
+  double gravity;   /* m/s2 */
+  double diffusion; /* del^4 coefficient, m4/s (0 = none) */
+} lmw_sphere_params;
+
+/* reference PlaneParams, cpp/plane.hpp:9-18 */
+typedef struct {
+  int nx, ny;
+  double lx, ly;
+  double gravity;
+  double depth;
+  double f0, beta;
+  double viscosity;
+  double drag;
+  int linear;
+} lmw_plane_params;
+
+typedef struct lmw_sphere lmw_sphere;
+typedef struct lmw_plane lmw_plane;
+typedef struct lmw_ensemble lmw_ensemble;
+
+/* ------------------------------------------------------------------------------------------- library */
+const char *lmw_last_error(void);
+/* device name, compute-unit count and device bytes held by live handles */
+lmw_status lmw_device_info(char *name, size_t name_len, int *compute_units, size_t *bytes_in_use);
+
+/* ------------------------------------------------------------------------------------------- free functions */
+/* replaces fft(std::vector<cplx>&, int sign), cpp/sphere.hpp:14 / sphere.cpp:21-45: `count` independent
+ * transforms of length n (a power of two, 2..4096), in place, unnormalized, sign -1 forward, +1 backward */
+lmw_status lmw_fft(double *re_im, size_t n, int count, int sign);
+/* replaces gauss_legendre(int, x, w), cpp/sphere.hpp:16 / sphere.cpp:47-64 (computed on the GPU) */
+lmw_status lmw_gauss_legendre(int n, double *x, double *w);
+
+/* ------------------------------------------------------------------------------------------- sphere */
+/* replaces Sphere::Sphere(int truncation, const SphereParams&), cpp/sphere.hpp:29 / sphere.cpp:75-103 */
+lmw_status lmw_sphere_create(int truncation, const lmw_sphere_params *params, lmw_sphere **out);
+lmw_status lmw_sphere_destroy(lmw_sphere *s);
+/* truncation(), nlat(), nlon(): cpp/sphere.hpp:30-32 */
+lmw_status lmw_sphere_dims(const lmw_sphere *s, int *truncation, int *nlat, int *nlon);
+/* arithmetic of the Legendre contractions (LMW_FP64 default) -- an addition, no reference counterpart */
+lmw_status lmw_sphere_set_precision(lmw_sphere *s, int precision);
+/* set_state_grid(h, u, v): cpp/sphere.hpp:39 / sphere.cpp:217-234 */
+lmw_status lmw_sphere_set_state_grid(lmw_sphere *s, const double *h, const double *u, const double *v, size_t n);
+/* set_orography_grid(hs): cpp/sphere.hpp:48 / sphere.cpp:105-109 */
+lmw_status lmw_sphere_set_orography_grid(lmw_sphere *s, const double *hs, size_t n);
+/* set_axis_tilt(alpha): cpp/sphere.hpp:40 / sphere.cpp:111-118 */
+lmw_status lmw_sphere_set_axis_tilt(lmw_sphere *s, double alpha);
+/* set_diffusion(nu4): cpp/sphere.hpp:41 */
+lmw_status lmw_sphere_set_diffusion(lmw_sphere *s, double nu4);
+/* mark_reference(): cpp/sphere.hpp:42 / sphere.cpp:236-239 */
+lmw_status lmw_sphere_mark_reference(lmw_sphere *s);
+/* add_mode(n, m, amplitude, mean_depth): cpp/sphere.hpp:43 / sphere.cpp:301-311 */
+lmw_status lmw_sphere_add_mode(lmw_sphere *s, int n, int m, double amplitude, double mean_depth);
+/* perturb(amplitude, seed, max_wave): cpp/sphere.hpp:44 / sphere.cpp:313-336 */
+lmw_status lmw_sphere_perturb(lmw_sphere *s, double amplitude, uint64_t seed, int max_wave);
+/* step(dt): cpp/sphere.hpp:63 / sphere.cpp:407-436 */
+lmw_status lmw_sphere_step(lmw_sphere *s, double dt);
+/* `count` steps of dt without the periodic checks of run() (the hot loop of sphere.cpp:471-472) */
+lmw_status lmw_sphere_steps(lmw_sphere *s, double dt, int64_t count);
+/* run(seconds, dt): cpp/sphere.hpp:64 / sphere.cpp:457-484 */
+lmw_status lmw_sphere_run(lmw_sphere *s, double seconds, double dt);
+/* stable_dt(safety): cpp/sphere.hpp:65 / sphere.cpp:438-447 */
+lmw_status lmw_sphere_stable_dt(lmw_sphere *s, double safety, double *out);
+/* grid_height/grid_u/grid_v/grid_vorticity/grid_tracer: cpp/sphere.hpp:91-94,55 / sphere.cpp:486-508 */
+lmw_status lmw_sphere_get_grid(lmw_sphere *s, int field, double *out, size_t n);
+/* the private spectral state zeta_, delta_, phi_ (cpp/sphere.hpp:109), reference layout */
+lmw_status lmw_sphere_get_spectral(lmw_sphere *s, int field, double *re_im, size_t n_complex);
+lmw_status lmw_sphere_set_spectral(lmw_sphere *s, int field, const double *re_im, size_t n_complex);
+/* mean_height/energy/kinetic_energy/potential_energy/potential_enstrophy/max_wind/height_error_l2/linf/time:
+ * cpp/sphere.hpp:70-81,34 / sphere.cpp:510-590 */
+lmw_status lmw_sphere_diag(lmw_sphere *s, int which, double *out);
+/* spectrum(ke, pe): cpp/sphere.hpp:78 / sphere.cpp:536-551 (n = truncation + 1 entries each) */
+lmw_status lmw_sphere_spectrum(lmw_sphere *s, double *ke, double *pe, size_t n);
+/* height_at/wind_at/vorticity_at: cpp/sphere.hpp:83-86 / sphere.cpp:592-637 (lat, lon in radians) */
+lmw_status lmw_sphere_point(lmw_sphere *s, int which, double lat, double lon, double *out);
+/* mode_coefficient(n, m, imag), wave_crest(m, lat): cpp/sphere.hpp:87-88 / sphere.cpp:639-652 */
+lmw_status lmw_sphere_mode_coefficient(lmw_sphere *s, int n, int m, int imag, double *out);
+lmw_status lmw_sphere_wave_crest(lmw_sphere *s, int m, double lat, double *out);
+/* the transforms exposed for tests: synthesize_scalar, analyze_scalar, analyze_divergence (cpp/sphere.hpp:97-100,
+ * sphere.cpp:138-177) and winds_spectral_to_grid (sphere.cpp:179-206) */
+lmw_status lmw_sphere_synthesize(lmw_sphere *s, const double *spec, double *grid);
+lmw_status lmw_sphere_analyze(lmw_sphere *s, const double *grid, double *spec);
+lmw_status lmw_sphere_analyze_divergence(lmw_sphere *s, const double *A, const double *B, double *spec);
+lmw_status lmw_sphere_winds(lmw_sphere *s, const double *zeta, const double *delta, double *U, double *V);
+/* one evaluation of tendencies(zeta, delta, phi) on the current state: cpp/sphere.cpp:338-390 */
+lmw_status lmw_sphere_tendencies(lmw_sphere *s, double *dzeta, double *ddelta, double *dphi);
+/* gauss_mu()/gauss_weights() (cpp/sphere.hpp:95-96) and the tables P_, H_ [lat][m][n-m] (cpp/sphere.hpp:108),
+ * returned as nlat x (N+1) x (N+1) doubles */
+lmw_status lmw_sphere_gauss(lmw_sphere *s, double *mu, double *w);
+lmw_status lmw_sphere_get_tables(lmw_sphere *s, double *P, double *H);
+
+/* ------------------------------------------------------------------------------------------- ensembles */
+/* An addition with no single reference counterpart: `members` independent Sphere objects (as many independent
+ * wx.sphere objects in one run, reference tests/determinism.lms:3-6) with possibly different truncations,
+ * stepped together.  Members of one truncation share one set of tables and one kernel launch per stage. */
+lmw_status lmw_ensemble_create(int members, const int *truncations, const lmw_sphere_params *params, int precision, lmw_ensemble **out);
+lmw_status lmw_ensemble_destroy(lmw_ensemble *e);
+lmw_status lmw_ensemble_member_dims(const lmw_ensemble *e, int member, int *truncation, int *nlat, int *nlon);
+lmw_status lmw_ensemble_set_state_grid(lmw_ensemble *e, int member, const double *h, const double *u, const double *v, size_t n);
+lmw_status lmw_ensemble_set_spectral(lmw_ensemble *e, int member, int field, const double *re_im, size_t n_complex);
+lmw_status lmw_ensemble_get_spectral(lmw_ensemble *e, int member, int field, double *re_im, size_t n_complex);
+lmw_status lmw_ensemble_get_grid(lmw_ensemble *e, int member, int field, double *out, size_t n);
+lmw_status lmw_ensemble_diag(lmw_ensemble *e, int member, int which, double *out);
+/* advance every member by `count` steps of its truncation's dt (dts[t] for the t-th distinct truncation in
+ * ascending order; lmw_ensemble_groups lists them) */
+lmw_status lmw_ensemble_groups(const lmw_ensemble *e, int *ngroups, int *truncations, int *members_per_group);
+lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t *counts);
+/* capture one RK4 step of every group into a HIP graph and replay it (hipGraph path); counts per group */
+lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts);
+
+/* ------------------------------------------------------------------------------------------- plane */
+/* replaces Plane::Plane(const PlaneParams&), cpp/plane.hpp:22 / plane.cpp:20-25; `batch` independent members
+ * share the parameters; precision LMW_FP64 or LMW_FP32 (state and arithmetic) */
+lmw_status lmw_plane_create(const lmw_plane_params *params, int batch, int precision, lmw_plane **out);
+lmw_status lmw_plane_destroy(lmw_plane *p);
+/* params() setters (cpp/plane.hpp:24, the Plane.gravity/depth/coriolis/linear/viscosity/drag operators of
+ * cpp/main.cpp:387-430); nx, ny, lx, ly are fixed at creation */
+lmw_status lmw_plane_set_params(lmw_plane *p, const lmw_plane_params *params);
+/* reset(): cpp/plane.hpp:39 / plane.cpp:27-32 (all members) */
+lmw_status lmw_plane_reset(lmw_plane *p);
+/* perturb(amplitude, seed): cpp/plane.hpp:40 / plane.cpp:34-57 (one member) */
+lmw_status lmw_plane_perturb(lmw_plane *p, int member, double amplitude, uint64_t seed);
+/* set_h/u/v_from_grid: cpp/plane.hpp:57-59 / plane.cpp:312-318 and the accessors h/u/v(i, j) */
+lmw_status lmw_plane_set_field(lmw_plane *p, int member, int field, const double *g, size_t n);
+lmw_status lmw_plane_get_field(lmw_plane *p, int member, int field, double *out, size_t n);
+/* step(dt): cpp/plane.hpp:41 / plane.cpp:143-158 (all members) */
+lmw_status lmw_plane_step(lmw_plane *p, double dt);
+lmw_status lmw_plane_steps(lmw_plane *p, double dt, int64_t count);
+/* run(seconds, dt): cpp/plane.hpp:42 / plane.cpp:160-189 (all members; the limit is the smallest over members) */
+lmw_status lmw_plane_run(lmw_plane *p, double seconds, double dt);
+/* stable_dt(safety): cpp/plane.hpp:43 / plane.cpp:131-141 (the smallest over members) */
+lmw_status lmw_plane_stable_dt(lmw_plane *p, double safety, double *out);
+/* mean_depth/energy/max_speed/potential_vorticity_anomaly_rms/time: cpp/plane.hpp:45-48,25 / plane.cpp:191-243 */
+lmw_status lmw_plane_diag(lmw_plane *p, int member, int which, double *out);
+/* one evaluation of tendency(h, u, v) for one member: cpp/plane.cpp:59-129 */
+lmw_status lmw_plane_tendency(lmw_plane *p, int member, double *dh, double *du, double *dv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

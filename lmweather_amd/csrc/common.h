@@ -1,0 +1,106 @@
+// Shared host/device helpers of the gfx950 dynamical core: error translation for the C ABI, device buffers,
+// complex arithmetic on double2.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lmw.h"
+
+namespace lmw {
+
+// An error that crosses the C ABI as (status, message).
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void invalid(const std::string &m) { throw Error(LMW_EINVAL, m); }
+[[noreturn]] inline void runtime(const std::string &m) { throw Error(LMW_ERUNTIME, m); }
+
+#define LMW_HIP(x)                                                                                          \
+  do {                                                                                                      \
+    hipError_t e_ = (x);                                                                                    \
+    if (e_ != hipSuccess) throw ::lmw::Error(LMW_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define LMW_LAUNCH_CHECK() LMW_HIP(hipGetLastError())
+
+void set_error(const std::string &msg);
+void note_bytes(int64_t delta);
+
+// Runs fn, mapping exceptions to a status and the thread-local message (no exception crosses the C ABI).
+template <class F> lmw_status guard(F &&fn) {
+  try {
+    fn();
+    return LMW_OK;
+  } catch (const Error &e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("out of host memory");
+    return LMW_ERUNTIME;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return LMW_ERUNTIME;
+  }
+}
+
+// std::to_string(double) formatting ("%f"), used in the reference's run() messages (sphere.cpp:465, plane.cpp:238).
+inline std::string fstr(double x) {
+  char b[64];
+  std::snprintf(b, sizeof b, "%f", x);
+  return b;
+}
+
+// Device array owned by a handle; zero-filled at allocation.
+template <class T> struct DevArray {
+  T *p = nullptr;
+  size_t n = 0;
+  DevArray() = default;
+  DevArray(const DevArray &) = delete;
+  DevArray &operator=(const DevArray &) = delete;
+  ~DevArray() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    LMW_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+    note_bytes(static_cast<int64_t>(count * sizeof(T)));
+    LMW_HIP(hipMemset(p, 0, count * sizeof(T)));
+  }
+  void release() {
+    if (p) {
+      (void)hipFree(p);
+      note_bytes(-static_cast<int64_t>(n * sizeof(T)));
+    }
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+  void upload(const T *src, size_t count, size_t offset, hipStream_t s) {
+    LMW_HIP(hipMemcpyAsync(p + offset, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void download(T *dst, size_t count, size_t offset, hipStream_t s) const {
+    LMW_HIP(hipMemcpyAsync(dst, p + offset, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// ---------------------------------------------------------------------------------- complex arithmetic
+__host__ __device__ inline double2 cx(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// (i m) * a, as (I * double(m)) * a of the reference: (-m a.im, m a.re)
+__host__ __device__ inline double2 cim(double m, double2 a) { return make_double2(-m * a.y, m * a.x); }
+// acc += a * s (complex times real)
+__device__ inline void cfma(double2 &acc, double2 a, double s) {
+  acc.x = fma(a.x, s, acc.x);
+  acc.y = fma(a.y, s, acc.y);
+}
+
+}  // namespace lmw

@@ -1,0 +1,110 @@
+// SphereCore: B members of one truncation resident on the GPU, stepped by the spectral transform method of the
+// reference (cpp/sphere.cpp:338-436) with the kernels in sphere_core.hip.
+//
+// Device data layout (all FP64 unless stated):
+//   spectral state      x0/xs/acc [b][field][S] double2, packed triangle: entry (m, n) at moff[m] + (n - m),
+//                        moff[m] = m (N+1) - m (m-1)/2, S = (N+1)(N+2)/2; fields zeta, delta, phi.
+//   Legendre tables     P, H for the NORTHERN latitudes only (P(-mu) = (-1)^(n-m) P(mu), H(-mu) = -(-1)^(n-m) H(mu)).
+//                        Within each m the rows are permuted: even n-m first (kpe[m] rows, padded to 16), then odd
+//                        (kpo[m] rows); row r of m is table row poff[m] + r.
+//                        "synthesis" layout Ps/Hs [row][jh] (jh fastest, stride Jhp);
+//                        "analysis"  layout Pa/Ha [m][jh][r]  (r fastest, stride kpe+kpo).
+//   Fourier coefficients Fs [b][j][m][4] double2: U, V (= u cos(lat), v cos(lat)), zeta, phi per latitude j.
+//   analysis operands    Gb [b][m][jh][16] double2: X+(4), X-(4), Y+(4), Y-(4) where X multiplies P and Y multiplies H
+//                        and +/- are the sums/differences of the northern and southern latitude of the pair jh.
+//   grids                f, hs, ref_phi [b][j][k]; diagnostic grids [b][4][j][k] = h, u, v, zeta.
+#pragma once
+#include <memory>
+
+#include "common.h"
+
+namespace lmw {
+
+struct Geo {
+  int N, M1, J, K, logK, Jh, Jhp, B, S, Tmax;
+  double a, g, omega;
+  const int *moff, *poff, *kpe, *kpo;
+};
+
+enum AnalMode { ANAL_TEND = 0, ANAL_STATE = 1, ANAL_RAW = 2, ANAL_TENDOUT = 3 };
+enum PrepMode { PREP_STATE = 0, PREP_DIV = 1, PREP_SCALAR = 2 };
+enum GridMode { GRID_DIAG = 0, GRID_RAW = 1 };
+
+struct SphereCore {
+  // geometry
+  int N, M1, J, K, logK, Jh, Jhp, S, B, R, Tmax;
+  lmw_sphere_params prm;
+  int precision = LMW_FP64;
+  hipStream_t stream = nullptr;
+  std::vector<int> h_moff, h_poff, h_kpe, h_kpo;
+  // per member host state
+  std::vector<double> t;                      // model time
+  std::vector<double> tilt;                   // axis tilt
+  std::vector<char> has_orography;
+  // device
+  DevArray<int> moff, poff, kpe, kpo;
+  DevArray<double> mu, w;
+  DevArray<double> Ps, Hs, Pa, Ha;             // FP64 tables
+  DevArray<float> Ps32, Hs32, Pa32, Ha32;      // FP32 tables (LMW_FP32)
+  DevArray<uint32_t> Psb, Hsb, Pab, Hab;       // split BF16 tables, hi|lo packed per entry (LMW_BF16X3)
+  DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K/2
+  DevArray<double2> x0, xs, acc;               // [b][3][S]
+  DevArray<double2> Fs;                        // [b][J][M1][4]
+  DevArray<double2> Gb;                        // [b][M1][Jh][16]
+  DevArray<double> f, hs, ref_phi;             // [b][J][K]
+  DevArray<double> grid;                       // [b][4][J][K]
+  DevArray<double> in3;                        // [b][3][J][K] input grids for the analysis path
+  DevArray<double2> spec_io;                   // [b][4][S] raw transform outputs / inputs
+  DevArray<double> red;                        // [b][16] reductions
+  hipGraphExec_t graph = nullptr;
+  double graph_dt = 0;
+
+  SphereCore(int truncation, const lmw_sphere_params &p, int members, int precision);
+  ~SphereCore();
+  SphereCore(const SphereCore &) = delete;
+  SphereCore &operator=(const SphereCore &) = delete;
+
+  Geo geo() const;
+  void set_precision(int p);
+  void sync();
+
+  // set-up (member b)
+  void set_axis_tilt(int b, double alpha);
+  void set_orography(int b, const double *hs_host);
+  void set_state_grid(int b, const double *h, const double *u, const double *v);
+  void set_spectral_packed(int b, int field, const double2 *packed);  // host packed triangle
+  void get_spectral_packed(int b, int field, double2 *packed);
+  void mark_reference(int b);
+
+  // stepping (all members)
+  void enqueue_step(double dt);
+  void steps(double dt, int64_t count);
+  void steps_graph(double dt, int64_t count);
+
+  // diagnostics
+  void compute_grids(int b0, int nb);                 // grid <- h, u, v, zeta of x0
+  void reduce(int b0, int nb);                        // red <- sums over grid
+  void download_grid(int b, int field, double *out);  // after compute_grids
+  void download_red(int b, double *out16);
+  double spectral_abs_sum(int b);
+
+  // raw transforms on member 0's scratch (exposed for tests)
+  void synthesize_raw(const double2 *z, const double2 *d, const double2 *phi, double *U, double *V, double *Z, double *PHI);
+  void analyze_raw(int mode, const double *g0, const double *g1, const double *g2, double2 *out4);
+  void tendencies_raw(int b, double2 *dz, double2 *dd, double2 *df);
+
+  // launches
+  void launch_synth(const double2 *spec, int b0, int nb);
+  void launch_tend_grid(int b0, int nb);
+  void launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
+  void launch_prep(int mode, int b0, int nb);
+  void launch_to_grid(int mode, double *out, int b0, int nb);
+
+  size_t packed_index(int m, int n) const { return static_cast<size_t>(h_moff[m] + (n - m)); }
+};
+
+// free functions on the GPU: batched FFT (host arrays in/out) and Gauss-Legendre quadrature
+void fft_batch(double2 *host, int n, int count, int sign);
+void gauss_device(int n, double *x, double *w);
+
+}  // namespace lmw

@@ -1,0 +1,935 @@
+// Spectral-transform shallow water on the sphere for gfx950: the reference's Sphere::tendencies / step
+// (cpp/sphere.cpp:338-436) as three kernels per RK4 stage, batched over the members of one truncation:
+//
+//   k_synth      inverse Legendre: per (m, 64 northern latitudes, 4 members) the even/odd-n partial sums of
+//                zeta P, phi P, psi P, chi P, psi H, chi H (sphere.cpp:150-162,179-206) folded by equatorial
+//                symmetry into the Fourier coefficients of U, V, zeta, phi at both latitudes of the pair.
+//   k_tend_grid  per (latitude pair, member) in LDS: two inverse complex FFTs per latitude (two real rows packed as
+//                one complex row), the grid products eta = zeta + f, A = U eta, B = V eta, U phi, V phi,
+//                phi + g hs + E (sphere.cpp:348-359), five forward FFTs (sphere.cpp:366-370) and the operands of
+//                the analysis with the Gaussian weights and the i m factors applied (sphere.cpp:371-376).
+//   k_anal       forward Legendre: per (m, 64 output rows, 4 members) the sums over northern latitudes of
+//                P X + H Y (sphere.cpp:377-384), then the RK4 stage update in the epilogue (sphere.cpp:416-434).
+//
+// FP64 throughout.  The FFT twiddles are exp(-2 pi i q / K) rounded once on the host.
+#include <cmath>
+
+#include "sphere_core.h"
+
+namespace lmw {
+
+namespace {
+constexpr int MB = 4;      // members per block in the Legendre kernels
+constexpr int CH = 32;     // latitudes staged per LDS chunk in k_anal
+constexpr int MAXM = 172;  // truncation <= 170 => at most 171 m values
+
+__device__ inline int brev(int k, int logK) { return static_cast<int>(__brev(static_cast<unsigned>(k)) >> (32 - logK)); }
+
+// eps(n, m) of sphere.cpp:11
+__host__ __device__ inline double eps_nm(int n, int m) {
+  return sqrt(static_cast<double>(n * n - m * m) / static_cast<double>(4 * n * n - 1));
+}
+
+// Iterative radix-2 DIT FFT over nf arrays of length K in LDS, data already in bit-reversed order.
+// tw[q] = exp(-2 pi i q / K); sign +1 uses the conjugate twiddles (backward transform, unnormalized).
+__device__ void fft_lds(double2 *buf, int nf, int K, int logK, const double2 *tw, int sign) {
+  const int halfK = K >> 1;
+  for (int s = 1; s <= logK; s++) {
+    const int half = 1 << (s - 1), tstride = K >> s;
+    for (int t = threadIdx.x; t < nf * halfK; t += blockDim.x) {
+      const int f = t >> (logK - 1), bt = t & (halfK - 1);
+      const int kk = bt & (half - 1), grp = bt >> (s - 1);
+      const int i0 = f * K + grp * 2 * half + kk, i1 = i0 + half;
+      double2 wv = tw[kk * tstride];
+      if (sign > 0) wv.y = -wv.y;
+      const double2 u = buf[i0], v = cmul(buf[i1], wv);
+      buf[i0] = cadd(u, v);
+      buf[i1] = csub(u, v);
+    }
+    __syncthreads();
+  }
+}
+
+// Real-pair unpacking: array holds the FFT of x + i y; returns the m-th coefficients X, Y divided by K
+// (fourier_forward's a[m] / nlon, sphere.cpp:128).
+__device__ inline void split_pair(const double2 *arr, int m, int K, double2 &X, double2 &Y) {
+  const double2 Zm = arr[m], Zc = arr[(K - m) & (K - 1)];
+  const double s = 0.5 / K;
+  X = cx((Zm.x + Zc.x) * s, (Zm.y - Zc.y) * s);
+  Y = cx((Zm.y + Zc.y) * s, (Zc.x - Zm.x) * s);
+}
+
+// Fills arr (length K, zeroed, bit-reversed order) with the spectrum of x + i y where x, y are the real rows with
+// Fourier coefficients X_m, Y_m, m = 0..N (fourier_backward, sphere.cpp:130-136: a[0] = Re F_0, a[K-m] = conj F_m).
+__device__ inline void pack_pair(double2 *arr, int m, int K, int logK, double2 X, double2 Y) {
+  if (m == 0) {
+    arr[0] = cx(X.x, Y.x);
+  } else {
+    arr[brev(m, logK)] = cx(X.x - Y.y, X.y + Y.x);
+    arr[brev(K - m, logK)] = cx(X.x + Y.y, Y.x - X.y);
+  }
+}
+
+// Writes the 16 analysis operands of one (m, latitude pair) from the per-latitude values (north n, south s).
+__device__ inline void store_operands(double2 *o, const double2 Xn[4], const double2 Xs[4], const double2 Yn[4], const double2 Ys[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    o[i] = cadd(Xn[i], Xs[i]);
+    o[4 + i] = csub(Xn[i], Xs[i]);
+    o[8 + i] = cadd(Yn[i], Ys[i]);
+    o[12 + i] = csub(Yn[i], Ys[i]);
+  }
+}
+}  // namespace
+
+// ------------------------------------------------------------------------------------------- set-up kernels
+
+// Gauss-Legendre nodes and weights (sphere.cpp:47-64), one thread per root pair.
+__global__ void k_gauss(int n, double *x, double *w) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (n + 1) / 2) return;
+  const double pi = 3.14159265358979323846;
+  double z = cos(pi * (i + 0.75) / (n + 0.5)), pp = 1;
+  for (int it = 0; it < 100; it++) {
+    double p1 = 1, p2 = 0;
+    for (int j = 1; j <= n; j++) {
+      double p3 = p2;
+      p2 = p1;
+      p1 = ((2.0 * j - 1) * z * p2 - (j - 1.0) * p3) / j;
+    }
+    pp = n * (z * p1 - p2) / (z * z - 1);
+    double dz = p1 / pp;
+    z -= dz;
+    if (fabs(dz) < 1e-15) break;
+  }
+  x[i] = -z;
+  x[n - 1 - i] = z;
+  w[i] = w[n - 1 - i] = 2 / ((1 - z * z) * pp * pp);
+}
+
+// Orthonormal P_n^m and H_n^m = (1 - mu^2) dP/dmu at the northern latitudes (sphere.cpp:66-73 and :84-96),
+// one thread per (m, latitude), the three-term recurrence kept in registers.
+__global__ void k_tables(Geo G, const double *mu, double *Ps, double *Hs, double *Pa, double *Ha) {
+#pragma clang fp contract(off)
+  const int m = blockIdx.x, jh = threadIdx.x;
+  if (jh >= G.Jh) return;
+  const double x = mu[G.Jh + jh];
+  const int kpe = G.kpe[m], kp = kpe + G.kpo[m];
+  const size_t base = static_cast<size_t>(G.poff[m]);
+  const double c = sqrt(fmax(0.0, 1 - x * x));
+  double p = 1 / sqrt(2.0);
+  for (int k = 1; k <= m; k++) p *= sqrt((2.0 * k + 1) / (2.0 * k)) * c;
+  double prev2 = 0.0, prev = p, cur = sqrt(2.0 * m + 3) * x * p;  // P_{n-1}, P_n, P_{n+1} at n = m
+  for (int n = m; n <= G.N; n++) {
+    const double minus = (n > m) ? prev2 : 0.0;
+    const double H = -n * eps_nm(n + 1, m) * cur + (n + 1) * eps_nm(n, m) * minus;
+    const int k = n - m, r = (k & 1) ? kpe + (k >> 1) : (k >> 1);
+    Ps[(base + r) * G.Jhp + jh] = prev;
+    Hs[(base + r) * G.Jhp + jh] = H;
+    Pa[base * G.Jhp + static_cast<size_t>(jh) * kp + r] = prev;
+    Ha[base * G.Jhp + static_cast<size_t>(jh) * kp + r] = H;
+    const double next = (n + 2 <= G.N + 1) ? (x * cur - eps_nm(n + 1, m) * prev) / eps_nm(n + 2, m) : 0.0;
+    prev2 = prev;
+    prev = cur;
+    cur = next;
+  }
+}
+
+// f = 2 Omega (-cos(lon) cos(lat) sin(alpha) + mu cos(alpha)) (sphere.cpp:111-118) for one member.
+__global__ void k_coriolis(Geo G, const double *mu, double *f, double alpha) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.x;
+  const double pi = 3.14159265358979323846;
+  const double c = sqrt(1 - mu[j] * mu[j]);
+  for (int k = threadIdx.x; k < G.K; k += blockDim.x) {
+    const double lam = 2 * pi * k / G.K;
+    f[static_cast<size_t>(j) * G.K + k] = 2 * G.omega * (-cos(lam) * c * sin(alpha) + mu[j] * cos(alpha));
+  }
+}
+
+// ------------------------------------------------------------------------------------------- the step kernels
+
+// Inverse Legendre transform of (zeta, delta, phi) to the Fourier coefficients of U, V, zeta, phi.
+__global__ __launch_bounds__(256) void k_synth(Geo G, const double *__restrict__ Ps, const double *__restrict__ Hs,
+                                               const double2 *__restrict__ spec, double2 *__restrict__ Fs, int b0, int nb) {
+  const int m = blockIdx.x, tx = threadIdx.x, ty = threadIdx.y;
+  const int jh = blockIdx.y * 64 + tx;
+  const int bl = blockIdx.z * MB + ty, b = b0 + bl;
+  const int Km = G.N - m + 1, Ke = (Km + 1) >> 1, Ko = Km >> 1;
+  __shared__ double2 cz[MB][MAXM], cf[MB][MAXM], cp[MB][MAXM], cc[MB][MAXM];
+  const double a = G.a;
+  if (bl < nb) {
+    const size_t off = static_cast<size_t>(G.moff[m]);
+    const double2 *sz = spec + (static_cast<size_t>(b) * 3 + 0) * G.S + off;
+    const double2 *sd = spec + (static_cast<size_t>(b) * 3 + 1) * G.S + off;
+    const double2 *sf = spec + (static_cast<size_t>(b) * 3 + 2) * G.S + off;
+    for (int k = tx; k < Km; k += 64) {
+      const int n = m + k;
+      const double kn = n >= 1 ? -a * a / (n * (n + 1.0)) : 0.0;  // psi, chi = -a^2/(n(n+1)) (zeta, delta): :185-190
+      const double2 z = sz[k], d = sd[k];
+      cz[ty][k] = z;
+      cf[ty][k] = sf[k];
+      cp[ty][k] = cscale(z, kn);
+      cc[ty][k] = cscale(d, kn);
+    }
+  }
+  __syncthreads();
+  if (bl >= nb || jh >= G.Jh) return;
+  double2 zE = cx(0, 0), fE = zE, pE = zE, cE = zE, pHE = zE, cHE = zE;
+  double2 zO = zE, fO = zE, pO = zE, cO = zE, pHO = zE, cHO = zE;
+  const size_t col = static_cast<size_t>(G.poff[m]) * G.Jhp + jh;
+  const double *pc = Ps + col, *hc = Hs + col;
+  const int kpe = G.kpe[m];
+  for (int r = 0; r < Ke; r++) {
+    const double P = pc[static_cast<size_t>(r) * G.Jhp], H = hc[static_cast<size_t>(r) * G.Jhp];
+    const int k = 2 * r;
+    cfma(zE, cz[ty][k], P);
+    cfma(fE, cf[ty][k], P);
+    cfma(pE, cp[ty][k], P);
+    cfma(cE, cc[ty][k], P);
+    cfma(pHE, cp[ty][k], H);
+    cfma(cHE, cc[ty][k], H);
+  }
+  for (int r = 0; r < Ko; r++) {
+    const double P = pc[static_cast<size_t>(kpe + r) * G.Jhp], H = hc[static_cast<size_t>(kpe + r) * G.Jhp];
+    const int k = 2 * r + 1;
+    cfma(zO, cz[ty][k], P);
+    cfma(fO, cf[ty][k], P);
+    cfma(pO, cp[ty][k], P);
+    cfma(cO, cc[ty][k], P);
+    cfma(pHO, cp[ty][k], H);
+    cfma(cHO, cc[ty][k], H);
+  }
+  const double dm = m;
+  for (int s = 0; s < 2; s++) {
+    const double sg = s == 0 ? 1.0 : -1.0;  // P even in mu for even n-m; H the opposite
+    const int j = s == 0 ? G.Jh + jh : G.Jh - 1 - jh;
+    const double2 Z = cadd(zE, cscale(zO, sg)), F = cadd(fE, cscale(fO, sg));
+    const double2 psP = cadd(pE, cscale(pO, sg)), chP = cadd(cE, cscale(cO, sg));
+    const double2 psH = cadd(cscale(pHE, sg), pHO), chH = cadd(cscale(cHE, sg), cHO);
+    // U_m = (i m chi P - psi H) / a, V_m = (i m psi P + chi H) / a  (sphere.cpp:200-201)
+    const double2 u = csub(cim(dm, chP), psH), v = cadd(cim(dm, psP), chH);
+    double2 *o = Fs + ((static_cast<size_t>(b) * G.J + j) * G.M1 + m) * 4;
+    o[0] = cx(u.x / a, u.y / a);
+    o[1] = cx(v.x / a, v.y / a);
+    o[2] = Z;
+    o[3] = F;
+  }
+}
+
+// Grid-point work of one latitude pair: inverse FFTs, nonlinear products, forward FFTs, analysis operands.
+__global__ __launch_bounds__(256) void k_tend_grid(Geo G, const double2 *__restrict__ Fs, const double *__restrict__ fgrid,
+                                                   const double *__restrict__ hs, const double2 *__restrict__ twg,
+                                                   const double *__restrict__ mu, const double *__restrict__ w,
+                                                   double2 *__restrict__ Gb, int b0) {
+  extern __shared__ double2 lds[];
+  const int K = G.K, logK = G.logK, M1 = G.M1, tid = threadIdx.x;
+  const int jh = blockIdx.x, b = b0 + blockIdx.y;
+  const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
+  double2 *buf = lds, *tw = lds + 5 * K;
+  for (int i = tid; i < K / 2; i += blockDim.x) tw[i] = twg[i];
+  for (int i = tid; i < 4 * K; i += blockDim.x) buf[i] = cx(0, 0);
+  __syncthreads();
+  for (int idx = tid; idx < 4 * M1; idx += blockDim.x) {
+    const int arr = idx / M1, m = idx - arr * M1;
+    const int j = (arr >> 1) == 0 ? jn : js, pr = arr & 1;
+    const double2 *src = Fs + ((static_cast<size_t>(b) * G.J + j) * M1 + m) * 4 + 2 * pr;
+    pack_pair(buf + arr * K, m, K, logK, src[0], src[1]);  // arrays: U+iV (n), zeta+i phi (n), U+iV (s), zeta+i phi (s)
+  }
+  __syncthreads();
+  fft_lds(buf, 4, K, logK, tw, +1);
+  const double om = 1 - mu[jn] * mu[jn];  // identical for the southern row: mu(s) = -mu(n)
+  const double g = G.g;
+  double2 r0[2], r1[2], r2[2], r3[2], r4[2];
+  {
+    int c = 0;
+    for (int k = tid; k < K; k += blockDim.x, c++) {
+      const size_t in = (static_cast<size_t>(b) * G.J + jn) * K + k, is = (static_cast<size_t>(b) * G.J + js) * K + k;
+      const double2 uvn = buf[k], zpn = buf[K + k], uvs = buf[2 * K + k], zps = buf[3 * K + k];
+      const double en = zpn.x + fgrid[in], es = zps.x + fgrid[is];
+      r0[c] = cx(uvn.x * en, uvn.y * en);
+      r1[c] = cx(uvn.x * zpn.y, uvn.y * zpn.y);
+      r2[c] = cx(uvs.x * es, uvs.y * es);
+      r3[c] = cx(uvs.x * zps.y, uvs.y * zps.y);
+      const double pen = zpn.y + g * hs[in] + (uvn.x * uvn.x + uvn.y * uvn.y) / (2 * om);
+      const double pes = zps.y + g * hs[is] + (uvs.x * uvs.x + uvs.y * uvs.y) / (2 * om);
+      r4[c] = cx(pen, pes);
+    }
+  }
+  __syncthreads();
+  {
+    int c = 0;
+    for (int k = tid; k < K; k += blockDim.x, c++) {
+      const int q = brev(k, logK);
+      buf[q] = r0[c];
+      buf[K + q] = r1[c];
+      buf[2 * K + q] = r2[c];
+      buf[3 * K + q] = r3[c];
+      buf[4 * K + q] = r4[c];
+    }
+  }
+  __syncthreads();
+  fft_lds(buf, 5, K, logK, tw, -1);
+  const double a = G.a, wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / a;
+  for (int m = tid; m < M1; m += blockDim.x) {
+    double2 An, Bn, UPn, VPn, As, Bs, UPs, VPs, PEn, PEs;
+    split_pair(buf, m, K, An, Bn);
+    split_pair(buf + K, m, K, UPn, VPn);
+    split_pair(buf + 2 * K, m, K, As, Bs);
+    split_pair(buf + 3 * K, m, K, UPs, VPs);
+    split_pair(buf + 4 * K, m, K, PEn, PEs);
+    const double dm = m;
+    // dzeta -= (i m A wd) P - (B wd) H; ddelta += (i m B wd) P + (A wd) H + (E w) P n(n+1)/a^2;
+    // dphi -= (i m U phi wd) P - (V phi wd) H   (sphere.cpp:374-383)
+    double2 Xn[4], Xs[4], Yn[4], Ys[4];
+    Xn[0] = cscale(cim(dm, An), -wd);
+    Xs[0] = cscale(cim(dm, As), -wd);
+    Yn[0] = cscale(Bn, wd);
+    Ys[0] = cscale(Bs, wd);
+    Xn[1] = cscale(cim(dm, Bn), wd);
+    Xs[1] = cscale(cim(dm, Bs), wd);
+    Yn[1] = cscale(An, wd);
+    Ys[1] = cscale(As, wd);
+    Xn[2] = cscale(PEn, wj);
+    Xs[2] = cscale(PEs, wj);
+    Yn[2] = cx(0, 0);
+    Ys[2] = cx(0, 0);
+    Xn[3] = cscale(cim(dm, UPn), -wd);
+    Xs[3] = cscale(cim(dm, UPs), -wd);
+    Yn[3] = cscale(VPn, wd);
+    Ys[3] = cscale(VPs, wd);
+    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, Xn, Xs, Yn, Ys);
+  }
+}
+
+// Grid inputs to analysis operands: the set_state_grid analyses (sphere.cpp:217-231), analyze_divergence
+// (:164-177) and analyze_scalar (:138-149).
+__global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__restrict__ in3, const double2 *__restrict__ twg,
+                                              const double *__restrict__ mu, const double *__restrict__ w, double2 *__restrict__ Gb, int b0) {
+  extern __shared__ double2 lds[];
+  const int K = G.K, logK = G.logK, M1 = G.M1, tid = threadIdx.x;
+  const int jh = blockIdx.x, b = b0 + blockIdx.y;
+  const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
+  double2 *buf = lds, *tw = lds + 5 * K;
+  const size_t plane = static_cast<size_t>(G.J) * K;
+  const double *g0 = in3 + static_cast<size_t>(blockIdx.y) * 3 * plane, *g1 = g0 + plane, *g2 = g1 + plane;
+  for (int i = tid; i < K / 2; i += blockDim.x) tw[i] = twg[i];
+  const double c = sqrt(1 - mu[jn] * mu[jn]);
+  const size_t rn = static_cast<size_t>(jn) * K, rs = static_cast<size_t>(js) * K;
+  int nf;
+  for (int k = tid; k < K; k += blockDim.x) {
+    const int q = brev(k, logK);
+    if (mode == PREP_STATE) {  // U = u cos, V = v cos, phi = g h (sphere.cpp:221-229)
+      buf[q] = cx(g1[rn + k] * c, g2[rn + k] * c);
+      buf[K + q] = cx(g1[rs + k] * c, g2[rs + k] * c);
+      buf[2 * K + q] = cx(G.g * g0[rn + k], G.g * g0[rs + k]);
+    } else if (mode == PREP_DIV) {
+      buf[q] = cx(g0[rn + k], g1[rn + k]);
+      buf[K + q] = cx(g0[rs + k], g1[rs + k]);
+    } else {
+      buf[q] = cx(g0[rn + k], g0[rs + k]);
+    }
+  }
+  nf = mode == PREP_STATE ? 3 : mode == PREP_DIV ? 2 : 1;
+  __syncthreads();
+  fft_lds(buf, nf, K, logK, tw, -1);
+  const double wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / G.a;
+  for (int m = tid; m < M1; m += blockDim.x) {
+    const double dm = m;
+    double2 Xn[4], Xs[4], Yn[4], Ys[4];
+    for (int i = 0; i < 4; i++) Xn[i] = Xs[i] = Yn[i] = Ys[i] = cx(0, 0);
+    if (mode == PREP_STATE) {
+      double2 Un, Vn, Us, Vs, Pn, Pss;
+      split_pair(buf, m, K, Un, Vn);
+      split_pair(buf + K, m, K, Us, Vs);
+      split_pair(buf + 2 * K, m, K, Pn, Pss);
+      // zeta = D(V, -U), delta = D(U, V) (sphere.cpp:211-212), phi by analyze_scalar
+      Xn[0] = cscale(cim(dm, Vn), wd);
+      Xs[0] = cscale(cim(dm, Vs), wd);
+      Yn[0] = cscale(Un, wd);
+      Ys[0] = cscale(Us, wd);
+      Xn[1] = cscale(cim(dm, Un), wd);
+      Xs[1] = cscale(cim(dm, Us), wd);
+      Yn[1] = cscale(Vn, -wd);
+      Ys[1] = cscale(Vs, -wd);
+      Xn[3] = cscale(Pn, wj);
+      Xs[3] = cscale(Pss, wj);
+    } else if (mode == PREP_DIV) {
+      double2 An, Bn, As, Bs;
+      split_pair(buf, m, K, An, Bn);
+      split_pair(buf + K, m, K, As, Bs);
+      Xn[0] = cscale(cim(dm, An), wd);
+      Xs[0] = cscale(cim(dm, As), wd);
+      Yn[0] = cscale(Bn, -wd);
+      Ys[0] = cscale(Bs, -wd);
+    } else {
+      double2 gn, gs;
+      split_pair(buf, m, K, gn, gs);
+      Xn[3] = cscale(gn, wj);
+      Xs[3] = cscale(gs, wj);
+    }
+    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, Xn, Xs, Yn, Ys);
+  }
+}
+
+// Forward Legendre transform and its epilogue (RK4 stage update, state write or raw output).
+__global__ __launch_bounds__(256) void k_anal(Geo G, const double *__restrict__ Pa, const double *__restrict__ Ha,
+                                              const double2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
+                                              double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
+                                              double2 *__restrict__ raw, int b0, int nb) {
+  const int m = blockIdx.x, tx = threadIdx.x, ty = threadIdx.y;
+  const int r = blockIdx.y * 64 + tx;
+  const int bl = blockIdx.z * MB + ty, b = b0 + bl;
+  const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Km = G.N - m + 1;
+  const bool odd = r >= kpe;
+  const int k = odd ? 2 * (r - kpe) + 1 : 2 * r;
+  const bool valid = (bl < nb) && (r < kp) && (k < Km);
+  __shared__ double2 sg[MB][CH][16];
+  double2 o1 = cx(0, 0), o2 = o1, o3 = o1, o4 = o1;
+  const size_t base = static_cast<size_t>(G.poff[m]) * G.Jhp + r;
+  const double *pc = Pa + base, *hc = Ha + base;
+  // even rows pair P with X+ and H with Y-, odd rows P with X- and H with Y+
+  const int xo = odd ? 4 : 0, yo = odd ? 8 : 12;
+  for (int j0 = 0; j0 < G.Jh; j0 += CH) {
+    const int cnt = min(CH, G.Jh - j0);
+    __syncthreads();
+    if (bl < nb) {
+      const double2 *src = Gb + ((static_cast<size_t>(b) * G.M1 + m) * G.Jh + j0) * 16;
+      for (int i = tx; i < cnt * 16; i += 64) sg[ty][i >> 4][i & 15] = src[i];
+    }
+    __syncthreads();
+    if (valid) {
+      for (int jj = 0; jj < cnt; jj++) {
+        const double P = pc[static_cast<size_t>(j0 + jj) * kp], H = hc[static_cast<size_t>(j0 + jj) * kp];
+        const double2 *gx = sg[ty][jj];
+        cfma(o1, gx[xo + 0], P);
+        cfma(o1, gx[yo + 0], H);
+        cfma(o2, gx[xo + 1], P);
+        cfma(o2, gx[yo + 1], H);
+        cfma(o3, gx[xo + 2], P);
+        cfma(o4, gx[xo + 3], P);
+        cfma(o4, gx[yo + 3], H);
+      }
+    }
+  }
+  if (!valid) return;
+  const int n = m + k;
+  const size_t q = static_cast<size_t>(G.moff[m]) + k;
+  const size_t S = G.S;
+  if (mode == ANAL_RAW) {  // raw outputs are indexed by the local member
+    raw[(static_cast<size_t>(bl) * 4 + 0) * S + q] = o1;
+    raw[(static_cast<size_t>(bl) * 4 + 1) * S + q] = o2;
+    raw[(static_cast<size_t>(bl) * 4 + 2) * S + q] = o3;
+    raw[(static_cast<size_t>(bl) * 4 + 3) * S + q] = o4;
+    return;
+  }
+  if (mode == ANAL_STATE) {
+    if (q == 0) o1 = o2 = cx(0, 0);  // the constant is not defined by the winds (sphere.cpp:213-214)
+    const size_t i0 = static_cast<size_t>(b) * 3 * S + q;
+    x0[i0] = o1;
+    x0[i0 + S] = o2;
+    x0[i0 + 2 * S] = o4;
+    xs[i0] = o1;
+    xs[i0 + S] = o2;
+    xs[i0 + 2 * S] = o4;
+    return;
+  }
+  double2 kz = o1, kd = cadd(o2, cscale(o3, n * (n + 1.0) / (G.a * G.a))), kf = o4;
+  if (q == 0) kz = kd = kf = cx(0, 0);  // sphere.cpp:387-389
+  if (mode == ANAL_TENDOUT) {  // tendencies only, indexed by the local member
+    raw[(static_cast<size_t>(bl) * 4 + 0) * S + q] = kz;
+    raw[(static_cast<size_t>(bl) * 4 + 1) * S + q] = kd;
+    raw[(static_cast<size_t>(bl) * 4 + 2) * S + q] = kf;
+    return;
+  }
+  {
+#pragma clang fp contract(off)
+    const double2 kk[3] = {kz, kd, kf};
+    double fv = 1.0, fpp = 1.0;
+    if (stage == 4 && nu4 > 0 && n >= 1) {  // del^4 filter (sphere.cpp:392-405)
+      const double a2 = G.a * G.a, l = n * (n + 1.0);
+      const double kv = (l - 2) / a2, kq = l / a2;
+      fv = exp(-nu4 * kv * kv * dt);
+      fpp = exp(-nu4 * kq * kq * dt);
+    }
+    for (int fld = 0; fld < 3; fld++) {
+      const size_t i = (static_cast<size_t>(b) * 3 + fld) * S + q;
+      const double2 s0 = x0[i], kv = kk[fld];
+      if (stage == 1) {
+        acc[i] = kv;
+        xs[i] = cadd(s0, cscale(kv, 0.5 * dt));
+      } else if (stage == 2) {
+        acc[i] = cadd(acc[i], cscale(kv, 2.0));
+        xs[i] = cadd(s0, cscale(kv, 0.5 * dt));
+      } else if (stage == 3) {
+        acc[i] = cadd(acc[i], cscale(kv, 2.0));
+        xs[i] = cadd(s0, cscale(kv, dt));
+      } else {
+        double2 nx = cadd(s0, cscale(cadd(acc[i], kv), dt / 6));
+        if (stage == 4 && nu4 > 0 && n >= 1) nx = cscale(nx, fld == 2 ? fpp : fv);
+        x0[i] = nx;
+        xs[i] = nx;
+      }
+    }
+  }
+}
+
+// Inverse FFTs of the four Fourier fields to the grid: diagnostics (h = phi/g, u = U/cos, v = V/cos, zeta;
+// sphere.cpp:486-508) or the raw rows U, V, zeta, phi.
+__global__ __launch_bounds__(256) void k_to_grid(Geo G, int mode, const double2 *__restrict__ Fs, const double2 *__restrict__ twg,
+                                                 const double *__restrict__ mu, double *__restrict__ out, int b0) {
+  extern __shared__ double2 lds[];
+  const int K = G.K, logK = G.logK, M1 = G.M1, tid = threadIdx.x;
+  const int jh = blockIdx.x, b = b0 + blockIdx.y;
+  const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
+  double2 *buf = lds, *tw = lds + 5 * K;
+  for (int i = tid; i < K / 2; i += blockDim.x) tw[i] = twg[i];
+  for (int i = tid; i < 4 * K; i += blockDim.x) buf[i] = cx(0, 0);
+  __syncthreads();
+  for (int idx = tid; idx < 4 * M1; idx += blockDim.x) {
+    const int arr = idx / M1, m = idx - arr * M1;
+    const int j = (arr >> 1) == 0 ? jn : js, pr = arr & 1;
+    const double2 *src = Fs + ((static_cast<size_t>(b) * G.J + j) * M1 + m) * 4 + 2 * pr;
+    pack_pair(buf + arr * K, m, K, logK, src[0], src[1]);
+  }
+  __syncthreads();
+  fft_lds(buf, 4, K, logK, tw, +1);
+  const double c = sqrt(1 - mu[jn] * mu[jn]);
+  const size_t plane = static_cast<size_t>(G.J) * K;
+  double *o = out + static_cast<size_t>(b) * 4 * plane;
+  for (int k = tid; k < K; k += blockDim.x) {
+    for (int s = 0; s < 2; s++) {
+      const int j = s == 0 ? jn : js;
+      const double2 uv = buf[2 * s * K + k], zp = buf[(2 * s + 1) * K + k];
+      const size_t i = static_cast<size_t>(j) * K + k;
+      if (mode == GRID_DIAG) {
+        o[i] = zp.y / G.g;
+        o[plane + i] = uv.x / c;
+        o[2 * plane + i] = uv.y / c;
+        o[3 * plane + i] = zp.x;
+      } else {
+        o[i] = uv.x;
+        o[plane + i] = uv.y;
+        o[2 * plane + i] = zp.x;
+        o[3 * plane + i] = zp.y;
+      }
+    }
+  }
+}
+
+namespace {
+// Neumaier-compensated accumulator
+struct Kahan {
+  double s = 0, c = 0;
+  __device__ void add(double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+  }
+  __device__ double value() const { return s + c; }
+};
+}  // namespace
+
+// Area means and extremes of one member (sphere.cpp:438-447,514-590), rows in parallel, rows combined in order.
+// red[b][0..8] = kinetic, potential, potential enstrophy, max wind, max h, err num, err den, max |h - ref|, max |ref|.
+__global__ __launch_bounds__(256) void k_reduce(Geo G, const double *__restrict__ grid, const double *__restrict__ fgrid,
+                                                const double *__restrict__ hs, const double *__restrict__ ref_phi,
+                                                const double *__restrict__ w, double *__restrict__ red, int b0) {
+  const int b = b0 + blockIdx.x, j = threadIdx.x, K = G.K;
+  __shared__ double rows[9][256];
+  const size_t plane = static_cast<size_t>(G.J) * K;
+  if (j < G.J) {
+    const double *hh = grid + static_cast<size_t>(b) * 4 * plane + static_cast<size_t>(j) * K;
+    const double *uu = hh + plane, *vv = hh + 2 * plane, *zz = hh + 3 * plane;
+    const double *ff = fgrid + static_cast<size_t>(b) * plane + static_cast<size_t>(j) * K;
+    const double *ss = hs + static_cast<size_t>(b) * plane + static_cast<size_t>(j) * K;
+    const double *rr = ref_phi + static_cast<size_t>(b) * plane + static_cast<size_t>(j) * K;
+    Kahan ke, pe, en, num, den;
+    double wmax = 0, hmax = 0, mn = 0, mr = 0;
+    const double g = G.g;
+    for (int k = 0; k < K; k++) {
+      const double h = hh[k], u = uu[k], v = vv[k], z = zz[k] + ff[k];
+      ke.add(0.5 * h * (u * u + v * v));
+      pe.add(0.5 * g * h * h + g * h * ss[k]);
+      en.add(0.5 * z * z / h);
+      wmax = fmax(wmax, hypot(u, v));
+      hmax = fmax(hmax, h);
+      const double rf = rr[k] / g, d = h - rf;
+      num.add(d * d);
+      den.add(rf * rf);
+      mn = fmax(mn, fabs(d));
+      mr = fmax(mr, fabs(rf));
+    }
+    rows[0][j] = ke.value();
+    rows[1][j] = pe.value();
+    rows[2][j] = en.value();
+    rows[3][j] = wmax;
+    rows[4][j] = hmax;
+    rows[5][j] = num.value();
+    rows[6][j] = den.value();
+    rows[7][j] = mn;
+    rows[8][j] = mr;
+  }
+  __syncthreads();
+  if (j < 9) {
+    // one thread per quantity, over all rows in order: 0..2 area means (x 1/2, sphere.cpp:520-533), 3, 4 maxima,
+    // 5, 6 weighted sums (sphere.cpp:579-580), 7, 8 maxima
+    double out;
+    if (j < 3 || j == 5 || j == 6) {
+      Kahan s;
+      for (int r = 0; r < G.J; r++) s.add(j < 3 ? w[r] * rows[j][r] / K : w[r] * rows[j][r]);
+      out = j < 3 ? 0.5 * s.value() : s.value();
+    } else {
+      out = 0;
+      for (int r = 0; r < G.J; r++) out = fmax(out, rows[j][r]);
+    }
+    red[static_cast<size_t>(b) * 16 + j] = out;
+  }
+}
+
+// Sum of |coefficient| over zeta, delta, phi of one member (the finiteness check of sphere.cpp:449-455).
+__global__ __launch_bounds__(256) void k_abs_sum(const double2 *__restrict__ x0, size_t count, double *__restrict__ out) {
+  __shared__ double part[256];
+  double s = 0;
+  for (size_t i = threadIdx.x; i < count; i += blockDim.x) s += hypot(x0[i].x, x0[i].y);
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = part[0];
+}
+
+// ------------------------------------------------------------------------------------------- host side
+
+SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, int prec) : prm(p), precision(LMW_FP64) {
+  if (truncation < 4 || truncation > 170) invalid("truncation must be between 4 and 170");
+  if (members < 1) invalid("an ensemble needs at least one member");
+  N = truncation;
+  K = 4;
+  while (K < 3 * N + 1) K *= 2;
+  J = K / 2;
+  logK = 0;
+  while ((1 << logK) < K) logK++;
+  Jh = J / 2;
+  Jhp = (Jh + 15) / 16 * 16;
+  M1 = N + 1;
+  S = M1 * (M1 + 1) / 2;
+  B = members;
+  h_moff.resize(M1);
+  h_poff.resize(M1);
+  h_kpe.resize(M1);
+  h_kpo.resize(M1);
+  R = 0;
+  for (int m = 0; m <= N; m++) {
+    const int Km = N - m + 1;
+    h_moff[m] = m * M1 - m * (m - 1) / 2;
+    h_kpe[m] = ((Km + 1) / 2 + 15) / 16 * 16;
+    h_kpo[m] = (Km / 2 + 15) / 16 * 16;
+    h_poff[m] = R;
+    R += h_kpe[m] + h_kpo[m];
+  }
+  Tmax = h_kpe[0] + h_kpo[0];
+  LMW_HIP(hipStreamCreate(&stream));
+  moff.alloc(M1);
+  poff.alloc(M1);
+  kpe.alloc(M1);
+  kpo.alloc(M1);
+  moff.upload(h_moff.data(), M1, 0, stream);
+  poff.upload(h_poff.data(), M1, 0, stream);
+  kpe.upload(h_kpe.data(), M1, 0, stream);
+  kpo.upload(h_kpo.data(), M1, 0, stream);
+  mu.alloc(J);
+  w.alloc(J);
+  const size_t tab = static_cast<size_t>(R) * Jhp;
+  Ps.alloc(tab);
+  Hs.alloc(tab);
+  Pa.alloc(tab);
+  Ha.alloc(tab);
+  std::vector<double2> htw(K / 2);
+  for (int q = 0; q < K / 2; q++) {
+    const long double ang = 2.0L * 3.141592653589793238462643383279502884L * q / K;
+    htw[q] = make_double2(static_cast<double>(cosl(ang)), static_cast<double>(-sinl(ang)));
+  }
+  tw.alloc(K / 2);
+  tw.upload(htw.data(), K / 2, 0, stream);
+  const size_t nstate = static_cast<size_t>(B) * 3 * S;
+  x0.alloc(nstate);
+  xs.alloc(nstate);
+  acc.alloc(nstate);
+  Fs.alloc(static_cast<size_t>(B) * J * M1 * 4);
+  Gb.alloc(static_cast<size_t>(B) * M1 * Jh * 16);
+  const size_t plane = static_cast<size_t>(J) * K;
+  f.alloc(B * plane);
+  hs.alloc(B * plane);
+  ref_phi.alloc(B * plane);
+  grid.alloc(B * 4 * plane);
+  in3.alloc(3 * plane);
+  spec_io.alloc(4 * static_cast<size_t>(S));
+  red.alloc(static_cast<size_t>(B) * 16);
+  t.assign(B, 0.0);
+  tilt.assign(B, 0.0);
+  has_orography.assign(B, 0);
+  Geo G = geo();
+  hipLaunchKernelGGL(k_gauss, dim3(1), dim3(256), 0, stream, J, mu.p, w.p);
+  LMW_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_tables, dim3(M1), dim3(Jhp), 0, stream, G, mu.p, Ps.p, Hs.p, Pa.p, Ha.p);
+  LMW_LAUNCH_CHECK();
+  for (int b = 0; b < B; b++) set_axis_tilt(b, 0.0);
+  set_precision(prec);
+  sync();
+}
+
+SphereCore::~SphereCore() {
+  if (graph) (void)hipGraphExecDestroy(graph);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+Geo SphereCore::geo() const {
+  Geo G;
+  G.N = N;
+  G.M1 = M1;
+  G.J = J;
+  G.K = K;
+  G.logK = logK;
+  G.Jh = Jh;
+  G.Jhp = Jhp;
+  G.B = B;
+  G.S = S;
+  G.Tmax = Tmax;
+  G.a = prm.radius;
+  G.g = prm.gravity;
+  G.omega = prm.omega;
+  G.moff = moff.p;
+  G.poff = poff.p;
+  G.kpe = kpe.p;
+  G.kpo = kpo.p;
+  return G;
+}
+
+void SphereCore::sync() { LMW_HIP(hipStreamSynchronize(stream)); }
+
+void SphereCore::set_axis_tilt(int b, double alpha) {
+  tilt[b] = alpha;
+  const size_t plane = static_cast<size_t>(J) * K;
+  hipLaunchKernelGGL(k_coriolis, dim3(J), dim3(256), 0, stream, geo(), mu.p, f.p + b * plane, alpha);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::set_orography(int b, const double *hsh) {
+  const size_t plane = static_cast<size_t>(J) * K;
+  hs.upload(hsh, plane, b * plane, stream);
+  has_orography[b] = 1;
+}
+
+void SphereCore::launch_synth(const double2 *spec, int b0, int nb) {
+  dim3 grd(M1, (Jh + 63) / 64, (nb + MB - 1) / MB), blk(64, MB);
+  hipLaunchKernelGGL(k_synth, grd, blk, 0, stream, geo(), Ps.p, Hs.p, spec, Fs.p, b0, nb);
+  LMW_LAUNCH_CHECK();
+}
+
+static size_t grid_lds_bytes(int K) { return (5 * static_cast<size_t>(K) + K / 2) * sizeof(double2); }
+
+void SphereCore::launch_tend_grid(int b0, int nb) {
+  hipLaunchKernelGGL(k_tend_grid, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs.p, f.p, hs.p, tw.p, mu.p, w.p, Gb.p, b0);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  dim3 grd(M1, (Tmax + 63) / 64, (nb + MB - 1) / MB), blk(64, MB);
+  hipLaunchKernelGGL(k_anal, grd, blk, 0, stream, geo(), Pa.p, Ha.p, Gb.p, mode, stage, dt, prm.diffusion, x0.p, xs.p, acc.p, raw_out, b0, nb);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::launch_prep(int mode, int b0, int nb) {
+  // in3 holds the input grids of members b0.. at offset 0 (k_prep indexes it by the local member)
+  hipLaunchKernelGGL(k_prep, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), mode, in3.p, tw.p, mu.p, w.p, Gb.p, b0);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::launch_to_grid(int mode, double *out, int b0, int nb) {
+  hipLaunchKernelGGL(k_to_grid, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), mode, Fs.p, tw.p, mu.p, out, b0);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::set_precision(int p) {
+  if (p != LMW_FP64) invalid("precision not available");
+  precision = p;
+}
+
+void SphereCore::set_state_grid(int b, const double *h, const double *u, const double *v) {
+  const size_t plane = static_cast<size_t>(J) * K;
+  in3.upload(h, plane, 0, stream);
+  in3.upload(u, plane, plane, stream);
+  in3.upload(v, plane, 2 * plane, stream);
+  launch_prep(PREP_STATE, b, 1);
+  launch_anal(ANAL_STATE, 0, 0.0, b, 1, nullptr);
+  t[b] = 0;
+  mark_reference(b);
+}
+
+void SphereCore::set_spectral_packed(int b, int field, const double2 *packed) {
+  const size_t off = (static_cast<size_t>(b) * 3 + field) * S;
+  x0.upload(packed, S, off, stream);
+  xs.upload(packed, S, off, stream);
+}
+
+void SphereCore::get_spectral_packed(int b, int field, double2 *packed) {
+  x0.download(packed, S, (static_cast<size_t>(b) * 3 + field) * S, stream);
+  sync();
+}
+
+void SphereCore::mark_reference(int b) {
+  const size_t plane = static_cast<size_t>(J) * K;
+  launch_synth(x0.p, b, 1);
+  launch_to_grid(GRID_RAW, grid.p, b, 1);  // rows of member b land at grid[b][0..3]
+  LMW_HIP(hipMemcpyAsync(ref_phi.p + b * plane, grid.p + (static_cast<size_t>(b) * 4 + 3) * plane, plane * sizeof(double),
+                         hipMemcpyDeviceToDevice, stream));
+}
+
+void SphereCore::enqueue_step(double dt) {
+  for (int stage = 1; stage <= 4; stage++) {
+    launch_synth(xs.p, 0, B);
+    launch_tend_grid(0, B);
+    launch_anal(ANAL_TEND, stage, dt, 0, B, nullptr);
+  }
+}
+
+void SphereCore::steps(double dt, int64_t count) {
+  for (int64_t i = 0; i < count; i++) enqueue_step(dt);
+}
+
+void SphereCore::steps_graph(double dt, int64_t count) {
+  if (count <= 0) return;
+  if (!graph || graph_dt != dt) {
+    if (graph) {
+      LMW_HIP(hipGraphExecDestroy(graph));
+      graph = nullptr;
+    }
+    hipGraph_t g;
+    LMW_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(dt);
+    LMW_HIP(hipStreamEndCapture(stream, &g));
+    LMW_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+    LMW_HIP(hipGraphDestroy(g));
+    graph_dt = dt;
+  }
+  for (int64_t i = 0; i < count; i++) LMW_HIP(hipGraphLaunch(graph, stream));
+}
+
+void SphereCore::compute_grids(int b0, int nb) {
+  launch_synth(x0.p, b0, nb);
+  launch_to_grid(GRID_DIAG, grid.p, b0, nb);
+}
+
+void SphereCore::reduce(int b0, int nb) {
+  hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(256), 0, stream, geo(), grid.p, f.p, hs.p, ref_phi.p, w.p, red.p, b0);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::download_grid(int b, int field, double *out) {
+  const size_t plane = static_cast<size_t>(J) * K;
+  grid.download(out, plane, (static_cast<size_t>(b) * 4 + field) * plane, stream);
+  sync();
+}
+
+void SphereCore::download_red(int b, double *out16) {
+  red.download(out16, 16, static_cast<size_t>(b) * 16, stream);
+  sync();
+}
+
+double SphereCore::spectral_abs_sum(int b) {
+  hipLaunchKernelGGL(k_abs_sum, dim3(1), dim3(256), 0, stream, x0.p + static_cast<size_t>(b) * 3 * S, static_cast<size_t>(3) * S,
+                     red.p + static_cast<size_t>(b) * 16 + 15);
+  LMW_LAUNCH_CHECK();
+  double s;
+  red.download(&s, 1, static_cast<size_t>(b) * 16 + 15, stream);
+  sync();
+  return s;
+}
+
+void SphereCore::synthesize_raw(const double2 *z, const double2 *d, const double2 *phi, double *U, double *V, double *Z, double *PHI) {
+  // member 0's Fourier and grid scratch; the input goes through spec_io laid out as one member's [3][S]
+  std::vector<double2> zero(S, make_double2(0, 0));
+  spec_io.upload(z ? z : zero.data(), S, 0, stream);
+  spec_io.upload(d ? d : zero.data(), S, S, stream);
+  spec_io.upload(phi ? phi : zero.data(), S, 2 * S, stream);
+  launch_synth(spec_io.p, 0, 1);
+  launch_to_grid(GRID_RAW, grid.p, 0, 1);
+  const size_t plane = static_cast<size_t>(J) * K;
+  if (U) grid.download(U, plane, 0, stream);
+  if (V) grid.download(V, plane, plane, stream);
+  if (Z) grid.download(Z, plane, 2 * plane, stream);
+  if (PHI) grid.download(PHI, plane, 3 * plane, stream);
+  sync();
+}
+
+void SphereCore::analyze_raw(int mode, const double *g0, const double *g1, const double *g2, double2 *out4) {
+  const size_t plane = static_cast<size_t>(J) * K;
+  if (g0) in3.upload(g0, plane, 0, stream);
+  if (g1) in3.upload(g1, plane, plane, stream);
+  if (g2) in3.upload(g2, plane, 2 * plane, stream);
+  launch_prep(mode, 0, 1);
+  launch_anal(ANAL_RAW, 0, 0.0, 0, 1, spec_io.p);
+  spec_io.download(out4, 4 * static_cast<size_t>(S), 0, stream);
+  sync();
+}
+
+void SphereCore::tendencies_raw(int b, double2 *dz, double2 *dd, double2 *df) {
+  launch_synth(x0.p, b, 1);
+  launch_tend_grid(b, 1);
+  launch_anal(ANAL_TENDOUT, 0, 0.0, b, 1, spec_io.p);
+  spec_io.download(dz, S, 0, stream);
+  spec_io.download(dd, S, S, stream);
+  spec_io.download(df, S, 2 * S, stream);
+  sync();
+}
+
+
+// ------------------------------------------------------------------------------------------- free functions
+
+// `count` independent in-place FFTs of length n in LDS (fft, sphere.cpp:21-45): one block per transform.
+__global__ __launch_bounds__(256) void k_fft_batch(double2 *data, int n, int logn, const double2 *__restrict__ twg, int sign) {
+  extern __shared__ double2 lds[];
+  double2 *buf = lds, *tw = lds + n;
+  double2 *row = data + static_cast<size_t>(blockIdx.x) * n;
+  for (int i = threadIdx.x; i < n / 2; i += blockDim.x) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) buf[brev(i, logn)] = row[i];
+  __syncthreads();
+  fft_lds(buf, 1, n, logn, tw, sign);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) row[i] = buf[i];
+}
+
+void fft_batch(double2 *host, int n, int count, int sign) {
+  int logn = 0;
+  while ((1 << logn) < n) logn++;
+  std::vector<double2> htw(n / 2);
+  for (int q = 0; q < n / 2; q++) {
+    const long double ang = 2.0L * 3.141592653589793238462643383279502884L * q / n;
+    htw[q] = make_double2(static_cast<double>(cosl(ang)), static_cast<double>(-sinl(ang)));
+  }
+  DevArray<double2> d, t;
+  d.alloc(static_cast<size_t>(n) * count);
+  t.alloc(n / 2);
+  LMW_HIP(hipMemcpy(d.p, host, d.bytes(), hipMemcpyHostToDevice));
+  LMW_HIP(hipMemcpy(t.p, htw.data(), t.bytes(), hipMemcpyHostToDevice));
+  const size_t lds = (static_cast<size_t>(n) + n / 2) * sizeof(double2);
+  hipLaunchKernelGGL(k_fft_batch, dim3(count), dim3(256), lds, 0, d.p, n, logn, t.p, sign);
+  LMW_LAUNCH_CHECK();
+  LMW_HIP(hipMemcpy(host, d.p, d.bytes(), hipMemcpyDeviceToHost));
+}
+
+void gauss_device(int n, double *x, double *w) {
+  DevArray<double> dx, dw;
+  dx.alloc(n);
+  dw.alloc(n);
+  hipLaunchKernelGGL(k_gauss, dim3((n + 255) / 256), dim3(256), 0, 0, n, dx.p, dw.p);
+  LMW_LAUNCH_CHECK();
+  LMW_HIP(hipMemcpy(x, dx.p, dx.bytes(), hipMemcpyDeviceToHost));
+  LMW_HIP(hipMemcpy(w, dw.p, dw.bytes(), hipMemcpyDeviceToHost));
+}
+
+}  // namespace lmw

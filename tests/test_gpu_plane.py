@@ -1,0 +1,116 @@
+"""GPU parity of the C-grid plane (reference cpp/plane.cpp) against the CPU oracle, through the C ABI.
+
+FP64 runs with FMA contraction off in the stencil and the RK4 update, so the tendency and the state agree with the
+oracle to the last few ulps (the arithmetic and its order are the reference's); FP32 is checked against FP64.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.helpers import max_rel, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lmw():
+    import lmweather_amd
+    return lmweather_amd
+
+
+def pair(lmw, oracle, nx=64, ny=32, lx=6.4e5, ly=3.2e5, depth=1000.0, f0=1e-4, beta=1.6e-11, visc=0.0, drag=0.0, linear=False,
+         batch=1, precision=0):
+    p = lmw.PlaneParams(nx, ny, lx, ly, 9.81, depth, f0, beta, visc, drag, linear)
+    return lmw.Plane(p, batch, precision), oracle.OraclePlane(nx, ny, lx, ly, 9.81, depth, f0, beta, visc, drag, linear)
+
+
+def seed_state(g, o, seed=0, member=0):
+    rng = np.random.default_rng(seed)
+    nx, ny = o.nx, o.ny
+    h = 1000.0 + rng.standard_normal((ny, nx))
+    u = 0.5 * rng.standard_normal((ny, nx))
+    v = 0.5 * rng.standard_normal((ny + 1, nx))
+    for m, kw in ((g, {"member": member}), (o, {})):
+        m.set_field(0, h, **kw)
+        m.set_field(1, u, **kw)
+        m.set_field(2, v, **kw)
+    return h, u, v
+
+
+@pytest.mark.parametrize("visc,drag,linear", [(0.0, 0.0, False), (50.0, 1e-6, False), (10.0, 0.0, True)])
+def test_tendency(lmw, oracle, visc, drag, linear):
+    g, o = pair(lmw, oracle, visc=visc, drag=drag, linear=linear)
+    seed_state(g, o, 1)
+    for a, b in zip(g.tendency(), o.tendency()):
+        assert max_rel(a, b) <= 1e-14
+
+
+def test_steps_fp64(lmw, oracle):
+    g, o = pair(lmw, oracle, visc=20.0, drag=1e-7)
+    seed_state(g, o, 2)
+    g.steps(30.0, 100)
+    for _ in range(100):
+        o.step(30.0)
+    for f in range(3):
+        assert max_rel(g.get_field(f), o.get_field(f)) <= 1e-13
+    assert abs(g.energy() - o.energy()) <= 1e-12 * o.energy()
+    assert abs(g.mean_depth() - o.mean_depth()) <= 1e-13 * o.mean_depth()
+    assert abs(g.max_speed() - o.max_speed()) <= 1e-12 * o.max_speed()
+    assert abs(g.potential_vorticity_anomaly_rms() - o.pv_anomaly_rms()) <= 1e-11 * o.pv_anomaly_rms()
+    assert abs(g.stable_dt() - o.stable_dt()) <= 1e-12 * o.stable_dt()
+
+
+def test_batch_members_independent(lmw, oracle):
+    g, o = pair(lmw, oracle, batch=3)
+    seed_state(g, o, 5, member=2)
+    g.steps(30.0, 10)
+    for _ in range(10):
+        o.step(30.0)
+    assert max_rel(g.get_field(0, member=2), o.get_field(0)) <= 1e-13
+    assert np.all(g.get_field(0, member=0) == 1000.0)  # the other members started at rest (reset) and stay there
+
+
+def test_fp32_against_fp64(lmw, oracle):
+    g64, o = pair(lmw, oracle)
+    g32, _ = pair(lmw, oracle, precision=1)
+    h, u, v = seed_state(g64, o, 3)
+    for f, x in enumerate((h, u, v)):
+        g32.set_field(f, x)
+    g64.steps(30.0, 50)
+    g32.steps(30.0, 50)
+    assert rel_l2(g32.get_field(0) - 1000.0, g64.get_field(0) - 1000.0) <= 1e-3
+    assert rel_l2(g32.get_field(1), g64.get_field(1)) <= 1e-4
+
+
+def test_gravity_wave_and_errors(lmw, oracle):
+    """tests/core.cpp:101-117: a linear standing gravity wave after half a period; mass; bad parameters."""
+    p = lmw.PlaneParams(100, 8, 1.0e6, 2.0e4, 9.81, 100.0, 0.0, 0.0, 0.0, 0.0, True)
+    pl = lmw.Plane(p)
+    k, c = 2 * math.pi / p.lx, math.sqrt(p.gravity * p.depth)
+    h = np.array([[p.depth + math.cos(k * pl.x_center(i)) for i in range(p.nx)] for _ in range(p.ny)])
+    pl.set_h_from_grid(h)
+    pl.run(math.pi / (c * k), 0)
+    assert abs(pl.h_at(0.5 * pl.dx(), 1e4) - p.depth + math.cos(k * 0.5 * pl.dx())) <= 2e-3
+    assert abs(pl.mean_depth() - p.depth) <= 1e-10
+    with pytest.raises(ValueError, match="grid must be at least 4 x 4"):
+        lmw.Plane(lmw.PlaneParams(2, 2, 1.0, 1.0))
+    g, o = pair(lmw, oracle)
+    seed_state(g, o, 4)
+    with pytest.raises(ValueError) as e1:
+        g.run(1e5, 1e4)
+    with pytest.raises(ValueError) as e2:
+        o.run(1e5, 1e4)
+    assert str(e1.value) == str(e2.value)
+
+
+def test_run_matches_oracle(lmw, oracle):
+    g, o = pair(lmw, oracle, visc=5.0)
+    seed_state(g, o, 6)
+    o.perturb(0.5, 9)
+    g.perturb(0.5, 9)
+    assert max_rel(g.get_field(0), o.get_field(0)) == 0.0
+    g.run(3600.0 * 3, 0)
+    o.run(3600.0 * 3, 0)
+    assert g.time() == o.time()
+    assert max_rel(g.get_field(0), o.get_field(0)) <= 1e-13
