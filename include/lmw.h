@@ -23,6 +23,9 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default) /* the library builds with hidden visibility: export exactly these */
+#endif
 
 typedef int lmw_status;
 #define LMW_OK 0
@@ -179,6 +182,8 @@ lmw_status lmw_ensemble_create(int members, const int *truncations, const lmw_sp
 lmw_status lmw_ensemble_destroy(lmw_ensemble *e);
 lmw_status lmw_ensemble_member_dims(const lmw_ensemble *e, int member, int *truncation, int *nlat, int *nlon);
 lmw_status lmw_ensemble_set_state_grid(lmw_ensemble *e, int member, const double *h, const double *u, const double *v, size_t n);
+lmw_status lmw_ensemble_set_orography_grid(lmw_ensemble *e, int member, const double *hs, size_t n);
+lmw_status lmw_ensemble_set_axis_tilt(lmw_ensemble *e, int member, double alpha);
 lmw_status lmw_ensemble_set_spectral(lmw_ensemble *e, int member, int field, const double *re_im, size_t n_complex);
 lmw_status lmw_ensemble_get_spectral(lmw_ensemble *e, int member, int field, double *re_im, size_t n_complex);
 lmw_status lmw_ensemble_get_grid(lmw_ensemble *e, int member, int field, double *out, size_t n);
@@ -189,6 +194,10 @@ lmw_status lmw_ensemble_groups(const lmw_ensemble *e, int *ngroups, int *truncat
 lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t *counts);
 /* capture one RK4 step of every group into a HIP graph and replay it (hipGraph path); counts per group */
 lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts);
+/* the same steps launched eagerly with a HIP event pair around every kernel, on the stream the kernels run on;
+ * kernel_ms[3 g + 0..2] = total milliseconds of the inverse Legendre, grid-point (FFT + products) and forward
+ * Legendre kernels of group g (measurement entry point, no reference counterpart) */
+lmw_status lmw_ensemble_steps_timed(lmw_ensemble *e, const double *dts, const int64_t *counts, double *kernel_ms);
 
 /* ------------------------------------------------------------------------------------------- plane */
 /* replaces Plane::Plane(const PlaneParams&), cpp/plane.hpp:22 / plane.cpp:20-25; `batch` independent members
@@ -217,6 +226,9 @@ lmw_status lmw_plane_diag(lmw_plane *p, int member, int which, double *out);
 /* one evaluation of tendency(h, u, v) for one member: cpp/plane.cpp:59-129 */
 lmw_status lmw_plane_tendency(lmw_plane *p, int member, double *dh, double *du, double *dv);
 
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
 #ifdef __cplusplus
 }
 #endif

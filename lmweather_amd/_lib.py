@@ -92,6 +92,8 @@ SIGNATURES = {
     "lmw_ensemble_destroy": [_vp],
     "lmw_ensemble_member_dims": [_vp, C.c_int, _ip, _ip, _ip],
     "lmw_ensemble_set_state_grid": [_vp, C.c_int, _dp, _dp, _dp, C.c_size_t],
+    "lmw_ensemble_set_orography_grid": [_vp, C.c_int, _dp, C.c_size_t],
+    "lmw_ensemble_set_axis_tilt": [_vp, C.c_int, C.c_double],
     "lmw_ensemble_set_spectral": [_vp, C.c_int, C.c_int, _dp, C.c_size_t],
     "lmw_ensemble_get_spectral": [_vp, C.c_int, C.c_int, _dp, C.c_size_t],
     "lmw_ensemble_get_grid": [_vp, C.c_int, C.c_int, _dp, C.c_size_t],
@@ -99,6 +101,7 @@ SIGNATURES = {
     "lmw_ensemble_groups": [_vp, _ip, _ip, _ip],
     "lmw_ensemble_steps": [_vp, _dp, _i64p],
     "lmw_ensemble_steps_graph": [_vp, _dp, _i64p],
+    "lmw_ensemble_steps_timed": [_vp, _dp, _i64p, _dp],
     "lmw_plane_create": [C.POINTER(PlaneParams), C.c_int, C.c_int, C.POINTER(_vp)],
     "lmw_plane_destroy": [_vp],
     "lmw_plane_set_params": [_vp, C.POINTER(PlaneParams)],

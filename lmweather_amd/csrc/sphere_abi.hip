@@ -620,6 +620,24 @@ lmw_status lmw_ensemble_set_state_grid(lmw_ensemble *e, int member, const double
     mv.c->sync();
   });
 }
+lmw_status lmw_ensemble_set_orography_grid(lmw_ensemble *e, int member, const double *hs, size_t n) {
+  return guard([&] {
+    NEED(e);
+    MemberView mv = e->view(member);
+    if (n != mv.ngrid()) invalid("grid size mismatch");
+    NEED(hs);
+    mv.c->set_orography(mv.b, hs);
+    mv.c->sync();
+  });
+}
+lmw_status lmw_ensemble_set_axis_tilt(lmw_ensemble *e, int member, double alpha) {
+  return guard([&] {
+    NEED(e);
+    MemberView mv = e->view(member);
+    mv.c->set_axis_tilt(mv.b, alpha);
+    mv.c->sync();
+  });
+}
 lmw_status lmw_ensemble_set_spectral(lmw_ensemble *e, int member, int field, const double *re_im, size_t n_complex) {
   return guard([&] {
     NEED(e);
@@ -690,6 +708,20 @@ lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t 
 }
 lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts) {
   return guard([&] { ensemble_steps(e, dts, counts, true); });
+}
+lmw_status lmw_ensemble_steps_timed(lmw_ensemble *e, const double *dts, const int64_t *counts, double *kernel_ms) {
+  return guard([&] {
+    NEED(e && dts && counts && kernel_ms);
+    for (size_t g = 0; g < e->groups.size(); g++) {
+      if (counts[g] < 0) invalid("step count must not be negative");
+      SphereCore &c = *e->groups[g];
+      double *ms = kernel_ms + 3 * g;
+      ms[0] = ms[1] = ms[2] = 0;
+      c.steps_timed(dts[g], counts[g], ms);  // groups one after another: each kernel is timed alone on the device
+      for (int b = 0; b < c.B; b++)
+        for (int64_t i = 0; i < counts[g]; i++) c.t[b] += dts[g];
+    }
+  });
 }
 
 }  // extern "C"

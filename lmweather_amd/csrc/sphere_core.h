@@ -80,6 +80,9 @@ struct SphereCore {
   void enqueue_step(double dt);
   void steps(double dt, int64_t count);
   void steps_graph(double dt, int64_t count);
+  // eager steps with a HIP event pair around every launch; ms[0..2] += total milliseconds of the inverse Legendre,
+  // grid-point and forward Legendre kernels
+  void steps_timed(double dt, int64_t count, double ms[3]);
 
   // diagnostics
   void compute_grids(int b0, int nb);                 // grid <- h, u, v, zeta of x0

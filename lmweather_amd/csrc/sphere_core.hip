@@ -30,16 +30,16 @@ __host__ __device__ inline double eps_nm(int n, int m) {
   return sqrt(static_cast<double>(n * n - m * m) / static_cast<double>(4 * n * n - 1));
 }
 
-// Iterative radix-2 DIT FFT over nf arrays of length K in LDS, data already in bit-reversed order.
-// tw[q] = exp(-2 pi i q / K); sign +1 uses the conjugate twiddles (backward transform, unnormalized).
-__device__ void fft_lds(double2 *buf, int nf, int K, int logK, const double2 *tw, int sign) {
-  const int halfK = K >> 1;
-  for (int s = 1; s <= logK; s++) {
-    const int half = 1 << (s - 1), tstride = K >> s;
-    for (int t = threadIdx.x; t < nf * halfK; t += blockDim.x) {
-      const int f = t >> (logK - 1), bt = t & (halfK - 1);
+// Iterative radix-2 DIT FFT over nf arrays of length L in LDS, data already in bit-reversed order.
+// tw[q] = exp(-2 pi i q / (L << tws)); sign +1 uses the conjugate twiddles (backward transform, unnormalized).
+__device__ void fft_lds(double2 *buf, int nf, int L, int logL, const double2 *tw, int tws, int sign) {
+  const int halfL = L >> 1;
+  for (int s = 1; s <= logL; s++) {
+    const int half = 1 << (s - 1), tstride = (L >> s) << tws;
+    for (int t = threadIdx.x; t < nf * halfL; t += blockDim.x) {
+      const int f = t >> (logL - 1), bt = t & (halfL - 1);
       const int kk = bt & (half - 1), grp = bt >> (s - 1);
-      const int i0 = f * K + grp * 2 * half + kk, i1 = i0 + half;
+      const int i0 = f * L + grp * 2 * half + kk, i1 = i0 + half;
       double2 wv = tw[kk * tstride];
       if (sign > 0) wv.y = -wv.y;
       const double2 u = buf[i0], v = cmul(buf[i1], wv);
@@ -50,24 +50,28 @@ __device__ void fft_lds(double2 *buf, int nf, int K, int logK, const double2 *tw
   }
 }
 
-// Real-pair unpacking: array holds the FFT of x + i y; returns the m-th coefficients X, Y divided by K
-// (fourier_forward's a[m] / nlon, sphere.cpp:128).
-__device__ inline void split_pair(const double2 *arr, int m, int K, double2 &X, double2 &Y) {
-  const double2 Zm = arr[m], Zc = arr[(K - m) & (K - 1)];
-  const double s = 0.5 / K;
-  X = cx((Zm.x + Zc.x) * s, (Zm.y - Zc.y) * s);
-  Y = cx((Zm.y + Zc.y) * s, (Zc.x - Zm.x) * s);
+// Real rows of length K = 2L are transformed one row per complex array of length L: the even samples are the real
+// parts and the odd samples the imaginary parts, so every row keeps the rounding of its own magnitude (as the
+// reference's separate transform of each row does, sphere.cpp:123-136).  tw[q] = exp(-2 pi i q / K), q < L.
+//
+// Forward: arr = FFT_L(x[2k] + i x[2k+1]); returns X_m / K for 0 <= m < L (fourier_forward: a[m] / nlon).
+__device__ inline double2 rfft_coef(const double2 *arr, int m, int L, const double2 *tw, double invK) {
+  const double2 Zm = arr[m], Zc = arr[(L - m) & (L - 1)];
+  const double2 Fe = cx(0.5 * (Zm.x + Zc.x), 0.5 * (Zm.y - Zc.y));  // (Zm + conj Zc) / 2: even samples
+  const double2 Fo = cx(0.5 * (Zm.y + Zc.y), 0.5 * (Zc.x - Zm.x));  // (Zm - conj Zc) / 2i: odd samples
+  return cscale(cadd(Fe, cmul(tw[m], Fo)), invK);
 }
 
-// Fills arr (length K, zeroed, bit-reversed order) with the spectrum of x + i y where x, y are the real rows with
-// Fourier coefficients X_m, Y_m, m = 0..N (fourier_backward, sphere.cpp:130-136: a[0] = Re F_0, a[K-m] = conj F_m).
-__device__ inline void pack_pair(double2 *arr, int m, int K, int logK, double2 X, double2 Y) {
-  if (m == 0) {
-    arr[0] = cx(X.x, Y.x);
-  } else {
-    arr[brev(m, logK)] = cx(X.x - Y.y, X.y + Y.x);
-    arr[brev(K - m, logK)] = cx(X.x + Y.y, Y.x - X.y);
-  }
+// Backward: entry m' (0 <= m' < L) of the half-length spectrum whose backward transform is x[2k] + i x[2k+1] for
+// the real row x_k = sum_m a_m exp(2 pi i m k / K) with a_0 = Re F_0, a_m = F_m, a_{K-m} = conj F_m (m = 1..N) and
+// zero elsewhere (fourier_backward, sphere.cpp:130-136).  F_m is read at src[4 m].
+__device__ inline double2 irfft_entry(const double2 *__restrict__ src, int mp, int L, int N, const double2 *tw) {
+  double2 lo = cx(0, 0), hi = cx(0, 0);  // a_{m'} and a_{m'+L}
+  if (mp == 0) lo = cx(src[0].x, 0.0);
+  else if (mp <= N) lo = src[4 * mp];
+  if (mp >= 1 && L - mp <= N) hi = cconj(src[4 * (L - mp)]);
+  const double2 E = cadd(lo, hi), O = cmul(csub(lo, hi), cconj(tw[mp]));
+  return cx(E.x - O.y, E.y + O.x);  // E + i O
 }
 
 // Writes the 16 analysis operands of one (m, latitude pair) from the per-latitude values (north n, south s).
@@ -219,157 +223,142 @@ __global__ __launch_bounds__(256) void k_synth(Geo G, const double *__restrict__
 }
 
 // Grid-point work of one latitude pair: inverse FFTs, nonlinear products, forward FFTs, analysis operands.
+// Rows in LDS: inverse U, V, zeta, phi and forward A, B, U phi, V phi, phi + g hs + E, each at (north, south).
 __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const double2 *__restrict__ Fs, const double *__restrict__ fgrid,
                                                    const double *__restrict__ hs, const double2 *__restrict__ twg,
                                                    const double *__restrict__ mu, const double *__restrict__ w,
                                                    double2 *__restrict__ Gb, int b0) {
   extern __shared__ double2 lds[];
-  const int K = G.K, logK = G.logK, M1 = G.M1, tid = threadIdx.x;
+  const int K = G.K, L = K >> 1, logL = G.logK - 1, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
-  double2 *buf = lds, *tw = lds + 5 * K;
-  for (int i = tid; i < K / 2; i += blockDim.x) tw[i] = twg[i];
-  for (int i = tid; i < 4 * K; i += blockDim.x) buf[i] = cx(0, 0);
+  double2 *buf = lds, *tw = lds + 10 * L;
+  for (int i = tid; i < L; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
-  for (int idx = tid; idx < 4 * M1; idx += blockDim.x) {
-    const int arr = idx / M1, m = idx - arr * M1;
-    const int j = (arr >> 1) == 0 ? jn : js, pr = arr & 1;
-    const double2 *src = Fs + ((static_cast<size_t>(b) * G.J + j) * M1 + m) * 4 + 2 * pr;
-    pack_pair(buf + arr * K, m, K, logK, src[0], src[1]);  // arrays: U+iV (n), zeta+i phi (n), U+iV (s), zeta+i phi (s)
+  for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
+    const int r = idx >> logL, mp = idx & (L - 1);
+    const int j = (r & 1) ? js : jn;
+    const double2 *src = Fs + (static_cast<size_t>(b) * G.J + j) * M1 * 4 + (r >> 1);
+    buf[r * L + brev(mp, logL)] = irfft_entry(src, mp, L, G.N, tw);
   }
   __syncthreads();
-  fft_lds(buf, 4, K, logK, tw, +1);
+  fft_lds(buf, 8, L, logL, tw, 1, +1);
   const double om = 1 - mu[jn] * mu[jn];  // identical for the southern row: mu(s) = -mu(n)
   const double g = G.g;
-  double2 r0[2], r1[2], r2[2], r3[2], r4[2];
-  {
-    int c = 0;
-    for (int k = tid; k < K; k += blockDim.x, c++) {
-      const size_t in = (static_cast<size_t>(b) * G.J + jn) * K + k, is = (static_cast<size_t>(b) * G.J + js) * K + k;
-      const double2 uvn = buf[k], zpn = buf[K + k], uvs = buf[2 * K + k], zps = buf[3 * K + k];
-      const double en = zpn.x + fgrid[in], es = zps.x + fgrid[is];
-      r0[c] = cx(uvn.x * en, uvn.y * en);
-      r1[c] = cx(uvn.x * zpn.y, uvn.y * zpn.y);
-      r2[c] = cx(uvs.x * es, uvs.y * es);
-      r3[c] = cx(uvs.x * zps.y, uvs.y * zps.y);
-      const double pen = zpn.y + g * hs[in] + (uvn.x * uvn.x + uvn.y * uvn.y) / (2 * om);
-      const double pes = zps.y + g * hs[is] + (uvs.x * uvs.x + uvs.y * uvs.y) / (2 * om);
-      r4[c] = cx(pen, pes);
+  double2 res[10];
+  const int k = tid;  // one pair of longitudes (2k, 2k+1) per thread: L <= 256
+  if (k < L) {
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const int j = s ? js : jn;
+      const double2 U = buf[(0 + s) * L + k], V = buf[(2 + s) * L + k], Z = buf[(4 + s) * L + k], P = buf[(6 + s) * L + k];
+      const size_t gi = (static_cast<size_t>(b) * G.J + j) * K + 2 * k;
+      double o[5][2];
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const double u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = e ? P.y : P.x;
+        const double eta = z + fgrid[gi + e];
+        o[0][e] = u * eta;
+        o[1][e] = v * eta;
+        o[2][e] = u * ph;
+        o[3][e] = v * ph;
+        o[4][e] = ph + g * hs[gi + e] + (u * u + v * v) / (2 * om);  // Bernoulli: g (h + hs) + E (sphere.cpp:357)
+      }
+#pragma unroll
+      for (int q = 0; q < 5; q++) res[2 * q + s] = cx(o[q][0], o[q][1]);
     }
   }
   __syncthreads();
-  {
-    int c = 0;
-    for (int k = tid; k < K; k += blockDim.x, c++) {
-      const int q = brev(k, logK);
-      buf[q] = r0[c];
-      buf[K + q] = r1[c];
-      buf[2 * K + q] = r2[c];
-      buf[3 * K + q] = r3[c];
-      buf[4 * K + q] = r4[c];
-    }
+  if (k < L) {
+    const int q = brev(k, logL);
+#pragma unroll
+    for (int r = 0; r < 10; r++) buf[r * L + q] = res[r];
   }
   __syncthreads();
-  fft_lds(buf, 5, K, logK, tw, -1);
-  const double a = G.a, wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / a;
+  fft_lds(buf, 10, L, logL, tw, 1, -1);
+  const double a = G.a, wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / a, invK = 1.0 / K;
   for (int m = tid; m < M1; m += blockDim.x) {
-    double2 An, Bn, UPn, VPn, As, Bs, UPs, VPs, PEn, PEs;
-    split_pair(buf, m, K, An, Bn);
-    split_pair(buf + K, m, K, UPn, VPn);
-    split_pair(buf + 2 * K, m, K, As, Bs);
-    split_pair(buf + 3 * K, m, K, UPs, VPs);
-    split_pair(buf + 4 * K, m, K, PEn, PEs);
+    double2 c[10];
+#pragma unroll
+    for (int r = 0; r < 10; r++) c[r] = rfft_coef(buf + r * L, m, L, tw, invK);
     const double dm = m;
     // dzeta -= (i m A wd) P - (B wd) H; ddelta += (i m B wd) P + (A wd) H + (E w) P n(n+1)/a^2;
-    // dphi -= (i m U phi wd) P - (V phi wd) H   (sphere.cpp:374-383)
-    double2 Xn[4], Xs[4], Yn[4], Ys[4];
-    Xn[0] = cscale(cim(dm, An), -wd);
-    Xs[0] = cscale(cim(dm, As), -wd);
-    Yn[0] = cscale(Bn, wd);
-    Ys[0] = cscale(Bs, wd);
-    Xn[1] = cscale(cim(dm, Bn), wd);
-    Xs[1] = cscale(cim(dm, Bs), wd);
-    Yn[1] = cscale(An, wd);
-    Ys[1] = cscale(As, wd);
-    Xn[2] = cscale(PEn, wj);
-    Xs[2] = cscale(PEs, wj);
-    Yn[2] = cx(0, 0);
-    Ys[2] = cx(0, 0);
-    Xn[3] = cscale(cim(dm, UPn), -wd);
-    Xs[3] = cscale(cim(dm, UPs), -wd);
-    Yn[3] = cscale(VPn, wd);
-    Ys[3] = cscale(VPs, wd);
-    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, Xn, Xs, Yn, Ys);
+    // dphi -= (i m U phi wd) P - (V phi wd) H   (sphere.cpp:374-383); index 0 north, 1 south
+    double2 X[2][4], Y[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const double2 A = c[0 + s], B = c[2 + s], UP = c[4 + s], VP = c[6 + s], PE = c[8 + s];
+      X[s][0] = cscale(cim(dm, A), -wd);
+      Y[s][0] = cscale(B, wd);
+      X[s][1] = cscale(cim(dm, B), wd);
+      Y[s][1] = cscale(A, wd);
+      X[s][2] = cscale(PE, wj);
+      Y[s][2] = cx(0, 0);
+      X[s][3] = cscale(cim(dm, UP), -wd);
+      Y[s][3] = cscale(VP, wd);
+    }
+    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, X[0], X[1], Y[0], Y[1]);
   }
 }
 
 // Grid inputs to analysis operands: the set_state_grid analyses (sphere.cpp:217-231), analyze_divergence
-// (:164-177) and analyze_scalar (:138-149).
+// (:164-177) and analyze_scalar (:138-149).  Rows: STATE U, V, phi; DIV A, B; SCALAR the field; each (north, south).
 __global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__restrict__ in3, const double2 *__restrict__ twg,
                                               const double *__restrict__ mu, const double *__restrict__ w, double2 *__restrict__ Gb, int b0) {
   extern __shared__ double2 lds[];
-  const int K = G.K, logK = G.logK, M1 = G.M1, tid = threadIdx.x;
+  const int K = G.K, L = K >> 1, logL = G.logK - 1, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
-  double2 *buf = lds, *tw = lds + 5 * K;
+  double2 *buf = lds, *tw = lds + 10 * L;
   const size_t plane = static_cast<size_t>(G.J) * K;
-  const double *g0 = in3 + static_cast<size_t>(blockIdx.y) * 3 * plane, *g1 = g0 + plane, *g2 = g1 + plane;
-  for (int i = tid; i < K / 2; i += blockDim.x) tw[i] = twg[i];
+  const double *g0 = in3 + static_cast<size_t>(blockIdx.y) * 3 * plane;
+  for (int i = tid; i < L; i += blockDim.x) tw[i] = twg[i];
   const double c = sqrt(1 - mu[jn] * mu[jn]);
-  const size_t rn = static_cast<size_t>(jn) * K, rs = static_cast<size_t>(js) * K;
-  int nf;
-  for (int k = tid; k < K; k += blockDim.x) {
-    const int q = brev(k, logK);
-    if (mode == PREP_STATE) {  // U = u cos, V = v cos, phi = g h (sphere.cpp:221-229)
-      buf[q] = cx(g1[rn + k] * c, g2[rn + k] * c);
-      buf[K + q] = cx(g1[rs + k] * c, g2[rs + k] * c);
-      buf[2 * K + q] = cx(G.g * g0[rn + k], G.g * g0[rs + k]);
-    } else if (mode == PREP_DIV) {
-      buf[q] = cx(g0[rn + k], g1[rn + k]);
-      buf[K + q] = cx(g0[rs + k], g1[rs + k]);
+  const int nrows = mode == PREP_STATE ? 6 : mode == PREP_DIV ? 4 : 2;
+  for (int idx = tid; idx < nrows * L; idx += blockDim.x) {
+    const int r = idx >> logL, k = idx & (L - 1);
+    const int field = r >> 1;
+    const size_t o = static_cast<size_t>((r & 1) ? js : jn) * K + 2 * k;
+    double x0, x1;
+    if (mode == PREP_STATE) {  // U = u cos, V = v cos, phi = g h (sphere.cpp:221-229); in3 = h, u, v
+      const double *src = g0 + (field == 2 ? 0 : (field + 1)) * plane;
+      const double f = field == 2 ? G.g : c;
+      x0 = field == 2 ? f * src[o] : src[o] * f;
+      x1 = field == 2 ? f * src[o + 1] : src[o + 1] * f;
     } else {
-      buf[q] = cx(g0[rn + k], g0[rs + k]);
+      const double *src = g0 + field * plane;
+      x0 = src[o];
+      x1 = src[o + 1];
     }
+    buf[r * L + brev(k, logL)] = cx(x0, x1);
   }
-  nf = mode == PREP_STATE ? 3 : mode == PREP_DIV ? 2 : 1;
   __syncthreads();
-  fft_lds(buf, nf, K, logK, tw, -1);
-  const double wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / G.a;
+  fft_lds(buf, nrows, L, logL, tw, 1, -1);
+  const double wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / G.a, invK = 1.0 / K;
   for (int m = tid; m < M1; m += blockDim.x) {
     const double dm = m;
-    double2 Xn[4], Xs[4], Yn[4], Ys[4];
-    for (int i = 0; i < 4; i++) Xn[i] = Xs[i] = Yn[i] = Ys[i] = cx(0, 0);
-    if (mode == PREP_STATE) {
-      double2 Un, Vn, Us, Vs, Pn, Pss;
-      split_pair(buf, m, K, Un, Vn);
-      split_pair(buf + K, m, K, Us, Vs);
-      split_pair(buf + 2 * K, m, K, Pn, Pss);
-      // zeta = D(V, -U), delta = D(U, V) (sphere.cpp:211-212), phi by analyze_scalar
-      Xn[0] = cscale(cim(dm, Vn), wd);
-      Xs[0] = cscale(cim(dm, Vs), wd);
-      Yn[0] = cscale(Un, wd);
-      Ys[0] = cscale(Us, wd);
-      Xn[1] = cscale(cim(dm, Un), wd);
-      Xs[1] = cscale(cim(dm, Us), wd);
-      Yn[1] = cscale(Vn, -wd);
-      Ys[1] = cscale(Vs, -wd);
-      Xn[3] = cscale(Pn, wj);
-      Xs[3] = cscale(Pss, wj);
-    } else if (mode == PREP_DIV) {
-      double2 An, Bn, As, Bs;
-      split_pair(buf, m, K, An, Bn);
-      split_pair(buf + K, m, K, As, Bs);
-      Xn[0] = cscale(cim(dm, An), wd);
-      Xs[0] = cscale(cim(dm, As), wd);
-      Yn[0] = cscale(Bn, -wd);
-      Ys[0] = cscale(Bs, -wd);
-    } else {
-      double2 gn, gs;
-      split_pair(buf, m, K, gn, gs);
-      Xn[3] = cscale(gn, wj);
-      Xs[3] = cscale(gs, wj);
+    double2 X[2][4], Y[2][4];
+    for (int s = 0; s < 2; s++)
+      for (int i = 0; i < 4; i++) X[s][i] = Y[s][i] = cx(0, 0);
+    for (int s = 0; s < 2; s++) {
+      if (mode == PREP_STATE) {
+        const double2 U = rfft_coef(buf + (0 + s) * L, m, L, tw, invK), V = rfft_coef(buf + (2 + s) * L, m, L, tw, invK);
+        const double2 P = rfft_coef(buf + (4 + s) * L, m, L, tw, invK);
+        // zeta = D(V, -U), delta = D(U, V) (sphere.cpp:211-212), phi by analyze_scalar
+        X[s][0] = cscale(cim(dm, V), wd);
+        Y[s][0] = cscale(U, wd);
+        X[s][1] = cscale(cim(dm, U), wd);
+        Y[s][1] = cscale(V, -wd);
+        X[s][3] = cscale(P, wj);
+      } else if (mode == PREP_DIV) {
+        const double2 A = rfft_coef(buf + (0 + s) * L, m, L, tw, invK), B = rfft_coef(buf + (2 + s) * L, m, L, tw, invK);
+        X[s][0] = cscale(cim(dm, A), wd);
+        Y[s][0] = cscale(B, -wd);
+      } else {
+        X[s][3] = cscale(rfft_coef(buf + s * L, m, L, tw, invK), wj);
+      }
     }
-    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, Xn, Xs, Yn, Ys);
+    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, X[0], X[1], Y[0], Y[1]);
   }
 }
 
@@ -480,40 +469,46 @@ __global__ __launch_bounds__(256) void k_anal(Geo G, const double *__restrict__ 
 __global__ __launch_bounds__(256) void k_to_grid(Geo G, int mode, const double2 *__restrict__ Fs, const double2 *__restrict__ twg,
                                                  const double *__restrict__ mu, double *__restrict__ out, int b0) {
   extern __shared__ double2 lds[];
-  const int K = G.K, logK = G.logK, M1 = G.M1, tid = threadIdx.x;
+  const int K = G.K, L = K >> 1, logL = G.logK - 1, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
-  double2 *buf = lds, *tw = lds + 5 * K;
-  for (int i = tid; i < K / 2; i += blockDim.x) tw[i] = twg[i];
-  for (int i = tid; i < 4 * K; i += blockDim.x) buf[i] = cx(0, 0);
+  double2 *buf = lds, *tw = lds + 10 * L;
+  for (int i = tid; i < L; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
-  for (int idx = tid; idx < 4 * M1; idx += blockDim.x) {
-    const int arr = idx / M1, m = idx - arr * M1;
-    const int j = (arr >> 1) == 0 ? jn : js, pr = arr & 1;
-    const double2 *src = Fs + ((static_cast<size_t>(b) * G.J + j) * M1 + m) * 4 + 2 * pr;
-    pack_pair(buf + arr * K, m, K, logK, src[0], src[1]);
+  for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
+    const int r = idx >> logL, mp = idx & (L - 1);
+    const int j = (r & 1) ? js : jn;
+    const double2 *src = Fs + (static_cast<size_t>(b) * G.J + j) * M1 * 4 + (r >> 1);
+    buf[r * L + brev(mp, logL)] = irfft_entry(src, mp, L, G.N, tw);
   }
   __syncthreads();
-  fft_lds(buf, 4, K, logK, tw, +1);
+  fft_lds(buf, 8, L, logL, tw, 1, +1);
   const double c = sqrt(1 - mu[jn] * mu[jn]);
   const size_t plane = static_cast<size_t>(G.J) * K;
   double *o = out + static_cast<size_t>(b) * 4 * plane;
-  for (int k = tid; k < K; k += blockDim.x) {
-    for (int s = 0; s < 2; s++) {
-      const int j = s == 0 ? jn : js;
-      const double2 uv = buf[2 * s * K + k], zp = buf[(2 * s + 1) * K + k];
-      const size_t i = static_cast<size_t>(j) * K + k;
-      if (mode == GRID_DIAG) {
-        o[i] = zp.y / G.g;
-        o[plane + i] = uv.x / c;
-        o[2 * plane + i] = uv.y / c;
-        o[3 * plane + i] = zp.x;
-      } else {
-        o[i] = uv.x;
-        o[plane + i] = uv.y;
-        o[2 * plane + i] = zp.x;
-        o[3 * plane + i] = zp.y;
-      }
+  for (int idx = tid; idx < 2 * L; idx += blockDim.x) {
+    const int s = idx >> logL, k = idx & (L - 1);
+    const int j = s ? js : jn;
+    const double2 U = buf[(0 + s) * L + k], V = buf[(2 + s) * L + k], Z = buf[(4 + s) * L + k], P = buf[(6 + s) * L + k];
+    const size_t i = static_cast<size_t>(j) * K + 2 * k;
+    if (mode == GRID_DIAG) {
+      o[i] = P.x / G.g;
+      o[i + 1] = P.y / G.g;
+      o[plane + i] = U.x / c;
+      o[plane + i + 1] = U.y / c;
+      o[2 * plane + i] = V.x / c;
+      o[2 * plane + i + 1] = V.y / c;
+      o[3 * plane + i] = Z.x;
+      o[3 * plane + i + 1] = Z.y;
+    } else {
+      o[i] = U.x;
+      o[i + 1] = U.y;
+      o[plane + i] = V.x;
+      o[plane + i + 1] = V.y;
+      o[2 * plane + i] = Z.x;
+      o[2 * plane + i + 1] = Z.y;
+      o[3 * plane + i] = P.x;
+      o[3 * plane + i + 1] = P.y;
     }
   }
 }
@@ -731,7 +726,7 @@ void SphereCore::launch_synth(const double2 *spec, int b0, int nb) {
   LMW_LAUNCH_CHECK();
 }
 
-static size_t grid_lds_bytes(int K) { return (5 * static_cast<size_t>(K) + K / 2) * sizeof(double2); }
+static size_t grid_lds_bytes(int K) { return 11 * static_cast<size_t>(K / 2) * sizeof(double2); }  // 10 rows + twiddles
 
 void SphereCore::launch_tend_grid(int b0, int nb) {
   hipLaunchKernelGGL(k_tend_grid, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs.p, f.p, hs.p, tw.p, mu.p, w.p, Gb.p, b0);
@@ -820,6 +815,31 @@ void SphereCore::steps_graph(double dt, int64_t count) {
   for (int64_t i = 0; i < count; i++) LMW_HIP(hipGraphLaunch(graph, stream));
 }
 
+void SphereCore::steps_timed(double dt, int64_t count, double ms[3]) {
+  if (count <= 0) return;
+  std::vector<hipEvent_t> ev(static_cast<size_t>(count) * 16);
+  for (auto &e : ev) LMW_HIP(hipEventCreate(&e));
+  size_t q = 0;
+  for (int64_t i = 0; i < count; i++)
+    for (int stage = 1; stage <= 4; stage++, q += 4) {
+      LMW_HIP(hipEventRecord(ev[q], stream));
+      launch_synth(xs.p, 0, B);
+      LMW_HIP(hipEventRecord(ev[q + 1], stream));
+      launch_tend_grid(0, B);
+      LMW_HIP(hipEventRecord(ev[q + 2], stream));
+      launch_anal(ANAL_TEND, stage, dt, 0, B, nullptr);
+      LMW_HIP(hipEventRecord(ev[q + 3], stream));
+    }
+  sync();
+  for (size_t k = 0; k < ev.size(); k += 4)
+    for (int j = 0; j < 3; j++) {
+      float t = 0;
+      LMW_HIP(hipEventElapsedTime(&t, ev[k + j], ev[k + j + 1]));
+      ms[j] += t;
+    }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+}
+
 void SphereCore::compute_grids(int b0, int nb) {
   launch_synth(x0.p, b0, nb);
   launch_to_grid(GRID_DIAG, grid.p, b0, nb);
@@ -899,7 +919,7 @@ __global__ __launch_bounds__(256) void k_fft_batch(double2 *data, int n, int log
   for (int i = threadIdx.x; i < n / 2; i += blockDim.x) tw[i] = twg[i];
   for (int i = threadIdx.x; i < n; i += blockDim.x) buf[brev(i, logn)] = row[i];
   __syncthreads();
-  fft_lds(buf, 1, n, logn, tw, sign);
+  fft_lds(buf, 1, n, logn, tw, 0, sign);
   for (int i = threadIdx.x; i < n; i += blockDim.x) row[i] = buf[i];
 }
 

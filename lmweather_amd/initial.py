@@ -44,16 +44,45 @@ class SplitMix:
         return s - 6.0
 
 
+def normals(seed: int, count: int, offset: int = 12345) -> np.ndarray:
+    """The first ``count`` Irwin-Hall normals of SplitMix(seed, offset), vectorized (the k-th splitmix state is
+    x0 + k * golden mod 2^64, so the whole stream is one array expression); summed left to right like the scalar
+    generator, so the values are identical."""
+    with np.errstate(over="ignore"):
+        x0 = np.uint64((seed * PHI_GOLDEN + offset) & MASK)
+        k = np.arange(1, 12 * count + 1, dtype=np.uint64)
+        z = x0 + k * np.uint64(PHI_GOLDEN)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    u = u.reshape(count, 12)
+    s = np.zeros(count)
+    for i in range(12):
+        s = s + u[:, i]
+    return s - 6.0
+
+
 def random_streamfunction(N: int, nmax: int, seed: int) -> np.ndarray:
-    """psi_nm = n^-1.5 (xi_re + i xi_im), 1 <= n <= nmax, xi_im = 0 at m = 0; complex (N+1, N+1) indexed [m, n]."""
-    rng = SplitMix(seed)
-    psi = np.zeros((N + 1, N + 1), dtype=np.complex128)
+    """psi_nm = n^-1.5 (xi_re + i xi_im), 1 <= n <= nmax, xi_im = 0 at m = 0; complex (N+1, N+1) indexed [m, n].
+    The normals are drawn in the reference perturb's order (m outer, n inner, re then im)."""
+    slots = []
     for m in range(nmax + 1):
         for n in range(max(m, 1), nmax + 1):
-            re = rng.normal()
-            im = 0.0 if m == 0 else rng.normal()
-            psi[m, n] = (re + 1j * im) * n ** -1.5
-    return psi
+            slots.append((m, n, 0))
+            if m:
+                slots.append((m, n, 1))
+    xi = normals(seed, len(slots))
+    psi = np.zeros((N + 1, N + 1), dtype=np.complex128)
+    for (m, n, part), x in zip(slots, xi):
+        if part == 0:
+            psi[m, n] += x
+        else:
+            psi[m, n] += 1j * x
+    n = np.arange(N + 1, dtype=np.float64)
+    scale = np.zeros(N + 1)
+    scale[1:] = n[1:] ** -1.5
+    return psi * scale[None, :]
 
 
 def _lap_factor(N: int, radius: float) -> np.ndarray:
@@ -112,13 +141,12 @@ def gaussian_orography(model, count: int = 8, seed: int = 2, hmin: float = 500.0
 
 
 def perturb_vorticity(zeta: np.ndarray, rel: float, seed: int) -> np.ndarray:
-    """zeta_nm (1 + rel xi_nm) with xi ~ N(0, 1) from the seeded stream, over the stored triangle n >= m."""
+    """zeta_nm (1 + rel xi_nm) with xi ~ N(0, 1) from the seeded stream, over the stored triangle n >= m (m outer)."""
     N = zeta.shape[0] - 1
-    rng = SplitMix(seed)
+    m_idx, n_idx = np.triu_indices(N + 1)
+    xi = normals(seed, m_idx.size)
     out = zeta.copy()
-    for m in range(N + 1):
-        for n in range(m, N + 1):
-            out[m, n] = zeta[m, n] * (1.0 + rel * rng.normal())
+    out[m_idx, n_idx] = zeta[m_idx, n_idx] * (1.0 + rel * xi)
     return out
 
 

@@ -289,6 +289,13 @@ class Ensemble:
         h, u, v = (L.as_f64(x).ravel() for x in (h, u, v))
         L.call("lmw_ensemble_set_state_grid", self._h, member, L.dptr(h), L.dptr(u), L.dptr(v), h.size)
 
+    def set_orography_grid(self, member, hs) -> None:
+        hs = L.as_f64(hs).ravel()
+        L.call("lmw_ensemble_set_orography_grid", self._h, member, L.dptr(hs), hs.size)
+
+    def set_axis_tilt(self, member, alpha) -> None:
+        L.call("lmw_ensemble_set_axis_tilt", self._h, member, alpha)
+
     def set_spectral(self, member, field, spec) -> None:
         s = L.as_c128(spec).ravel()
         L.call("lmw_ensemble_set_spectral", self._h, member, field, s.ctypes.data_as(L._dp), s.size)
@@ -316,6 +323,16 @@ class Ensemble:
         c = np.ascontiguousarray(counts, dtype=np.int64)
         fn = "lmw_ensemble_steps_graph" if graph else "lmw_ensemble_steps"
         L.call(fn, self._h, L.dptr(d), c.ctypes.data_as(L._i64p))
+
+
+    def steps_timed(self, dts, counts) -> np.ndarray:
+        """Eager steps with HIP events around every kernel; returns (groups, 3) total milliseconds of the inverse
+        Legendre, grid-point and forward Legendre kernels."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        c = np.ascontiguousarray(counts, dtype=np.int64)
+        ms = np.zeros(3 * len(self.group_truncations))
+        L.call("lmw_ensemble_steps_timed", self._h, L.dptr(d), c.ctypes.data_as(L._i64p), L.dptr(ms))
+        return ms.reshape(-1, 3)
 
 
 def fft(data, sign: int) -> np.ndarray:

@@ -8,9 +8,12 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(np.ravel(a - b)) / (den if den > 0 else 1.0))
 
 
-def max_rel(a, b) -> float:
+def max_rel(a, b, scale=None) -> float:
+    """max |a - b| relative to max |b| (or to ``scale`` when b is only round-off, e.g. the divergence of a
+    non-divergent flow)."""
     a, b = np.asarray(a), np.asarray(b)
-    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    s = np.max(np.abs(b)) if scale is None else scale
+    return float(np.max(np.abs(a - b)) / max(s, 1e-300))
 
 
 class OracleTransforms:

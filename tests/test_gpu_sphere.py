@@ -92,8 +92,9 @@ def test_state_and_tendencies(lmw, oracle, N):
     s, o, (h, u, v) = balanced(lmw, oracle, N, N // 2, 1)
     s.set_state_grid(h, u, v)
     o.set_state_grid(h, u, v)
-    for f in range(3):
-        assert max_rel(s.get_spectral(f), o.get_spectral(f)) <= 1e-12
+    zscale = np.max(np.abs(o.get_spectral(0)))
+    for f in range(3):  # delta is round-off only (the flow is non-divergent): measure it on the vorticity scale
+        assert max_rel(s.get_spectral(f), o.get_spectral(f), zscale if f == 1 else None) <= 1e-12
     t = s.tendencies()
     tr = o.tendencies()
     for a, b in zip(t, tr):
