@@ -97,6 +97,12 @@ __host__ __device__ inline double2 cmul(double2 a, double2 b) { return make_doub
 __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // (i m) * a, as (I * double(m)) * a of the reference: (-m a.im, m a.re)
 __host__ __device__ inline double2 cim(double m, double2 a) { return make_double2(-m * a.y, m * a.x); }
+// storage conversions of complex values (FP64 compute, FP64 or FP32 storage)
+__host__ __device__ inline double2 to_d2(double2 a) { return a; }
+__host__ __device__ inline double2 to_d2(float2 a) { return make_double2(a.x, a.y); }
+template <class T> __host__ __device__ inline T from_d2(double2 a);
+template <> __host__ __device__ inline double2 from_d2<double2>(double2 a) { return a; }
+template <> __host__ __device__ inline float2 from_d2<float2>(double2 a) { return make_float2(static_cast<float>(a.x), static_cast<float>(a.y)); }
 // acc += a * s (complex times real)
 __device__ inline void cfma(double2 &acc, double2 a, double s) {
   acc.x = fma(a.x, s, acc.x);

@@ -1,0 +1,405 @@
+// Mixed-precision Legendre transforms on MFMA for gfx950: FP32 operands (v_mfma_f32_16x16x4_f32) or split BF16
+// hi+lo operands with three products hi*hi + hi*lo + lo*hi (v_mfma_f32_16x16x32_bf16), FP32 accumulation.  The
+// Legendre tables are built in FP64 (k_tables) and rounded/split once; the state, the FFTs, the grid products and
+// the RK4 update stay FP64.
+//
+// Both transforms are, per zonal wavenumber m and parity of n - m, small GEMMs whose columns are ensemble members:
+//   inverse  F[jh][b] (quantity q) = sum_k' A[jh][k'] C_q[k'][b]      A = P or H rows of parity p, C = coefficients
+//   forward  O_q[r][b]            = sum_jh P[jh][r] X_q[jh][b] + H[jh][r] Y_q[jh][b]
+// Each 16 x 16 MFMA tile holds one quantity for 16 members, so one lane's accumulators carry every quantity of
+// its member and the epilogues (north/south combination, RK4 update) need no data exchange.
+#include <cmath>
+
+#include "sphere_core.h"
+#include "sphere_epilogue.h"
+
+namespace lmw {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+namespace {
+constexpr int MEM = 16;  // members per block (the N dimension of an MFMA tile)
+
+__device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+// acc += a * b for one 16x16 tile; PREC = LMW_FP32 consumes one float per lane, LMW_BF16X3 eight bf16 pairs
+__device__ inline v4f mma32(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ inline v4f mma3(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+  return c;
+}
+__device__ inline v8bf load8(const __bf16 *p) { return *reinterpret_cast<const v8bf *>(p); }
+}  // namespace
+
+// ------------------------------------------------------------------------------------------- table builders
+
+// Forward-transform A operand: FP32 [m][k = jh (P) | Jhp + jh (H)][r] and split BF16 [m][r][k], from the FP64
+// analysis tables Pa/Ha [m][jh][r].
+__global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa, const double *__restrict__ Ha, float *__restrict__ A32,
+                                    __bf16 *__restrict__ Abh, __bf16 *__restrict__ Abl, const int *__restrict__ mrow) {
+  const size_t total = static_cast<size_t>(2 * G.Jhp) * R;
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    // e enumerates (global row gr, k) with k fastest: gr = poff[m] + r
+    const int gr = static_cast<int>(e / (2 * G.Jhp)), k = static_cast<int>(e % (2 * G.Jhp));
+    const int m = mrow[gr], r = gr - G.poff[m], kp = G.kpe[m] + G.kpo[m];
+    const int jh = k < G.Jhp ? k : k - G.Jhp;
+    const double *T = k < G.Jhp ? Pa : Ha;
+    const float v = static_cast<float>(T[static_cast<size_t>(G.poff[m]) * G.Jhp + static_cast<size_t>(jh) * kp + r]);
+    A32[static_cast<size_t>(2 * G.Jhp) * G.poff[m] + static_cast<size_t>(k) * kp + r] = v;
+    __bf16 hi, lo;
+    split_bf16(v, hi, lo);
+    Abh[e] = hi;
+    Abl[e] = lo;
+  }
+}
+
+// Inverse-transform A operands: FP32 Ps32/Hs32 in the layout of Ps/Hs ([row][jh]), and split BF16 [m][p][jh][k']
+// with each parity block padded to 32 rows (soff[m] is the offset of m, ke32/ko32 the padded row counts).
+__global__ void k_build_synth_tables(Geo G, int R, const double *__restrict__ Ps, const double *__restrict__ Hs, float *__restrict__ Ps32,
+                                     float *__restrict__ Hs32, const int *__restrict__ soff, const int *__restrict__ ke32,
+                                     const int *__restrict__ ko32, __bf16 *__restrict__ Sph, __bf16 *__restrict__ Spl, __bf16 *__restrict__ Shh,
+                                     __bf16 *__restrict__ Shl) {
+  const size_t total = static_cast<size_t>(R) * G.Jhp;
+  for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    Ps32[e] = static_cast<float>(Ps[e]);
+    Hs32[e] = static_cast<float>(Hs[e]);
+  }
+  // bf16: one thread per (m, p, jh, k')
+  for (int m = blockIdx.x; m < G.M1; m += gridDim.x) {
+    for (int p = 0; p < 2; p++) {
+      const int kp32 = p ? ko32[m] : ke32[m], kp16 = p ? G.kpo[m] : G.kpe[m], rbase = p ? G.kpe[m] : 0;
+      const size_t base = static_cast<size_t>(soff[m]) + (p ? static_cast<size_t>(G.Jhp) * ke32[m] : 0);
+      for (int t = threadIdx.x; t < G.Jhp * kp32; t += blockDim.x) {
+        const int jh = t / kp32, kk = t % kp32;
+        float vp = 0, vh = 0;
+        if (kk < kp16) {
+          const size_t src = (static_cast<size_t>(G.poff[m]) + rbase + kk) * G.Jhp + jh;
+          vp = static_cast<float>(Ps[src]);
+          vh = static_cast<float>(Hs[src]);
+        }
+        __bf16 hi, lo;
+        split_bf16(vp, hi, lo);
+        Sph[base + t] = hi;
+        Spl[base + t] = lo;
+        split_bf16(vh, hi, lo);
+        Shh[base + t] = hi;
+        Shl[base + t] = lo;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------- inverse transform
+
+struct SynthTabs {
+  const float *Ps32, *Hs32;
+  const __bf16 *Sph, *Spl, *Shh, *Shl;
+  const int *soff, *ke32, *ko32;
+};
+
+// Block: one m, 16 members, one wave per 16 northern latitudes.  Coefficients are staged in LDS in chunks of 32
+// rows of one parity: quantities q = zeta re/im, phi re/im, psi re/im, chi re/im (H uses psi, chi = q 4..7).
+// LDS per chunk: FP32 [q][k'][member] floats; BF16 [q][member][k'] hi and lo.
+template <int PREC>
+__global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
+                                                  int nb, int mixed) {
+  constexpr int SP = 40;  // padded row pitch (bf16) of the BF16 chunk: distinct LDS banks across members
+  extern __shared__ float lds_f[];
+  float *cf = lds_f;                                                       // FP32 chunk [q][k'][member]
+  __bf16 *ch = reinterpret_cast<__bf16 *>(lds_f), *cl = ch + 8 * MEM * SP;  // BF16 chunk [q][member][k'], hi and lo
+  const int m = blockIdx.x, mb = blockIdx.y * MEM;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int Km = G.N - m + 1, kpe = G.kpe[m], kpo = G.kpo[m];
+  const double a = G.a;
+  const int jt = wave;  // row tile (16 northern latitudes) of this wave
+  const bool active = jt * 16 < G.Jh;
+  v4f accP[2][8], accH[2][4];
+  for (int p = 0; p < 2; p++) {
+    for (int q = 0; q < 8; q++) accP[p][q] = v4f{0, 0, 0, 0};
+    for (int q = 0; q < 4; q++) accH[p][q] = v4f{0, 0, 0, 0};
+  }
+  for (int p = 0; p < 2; p++) {
+    const int kp = p ? kpo : kpe;             // rows of this parity (16-padded)
+    const int kp32 = PREC == LMW_BF16X3 ? (p ? T.ko32[m] : T.ke32[m]) : kp;
+    for (int c0 = 0; c0 < kp32; c0 += 32) {
+      __syncthreads();
+      // stage 8 quantities x 32 rows x 16 members: one thread per (member, pair of rows), members fastest, so
+      // that a wavefront writes consecutive LDS words (FP32) or whole words of a padded pitch (BF16)
+      for (int e = threadIdx.x; e < 16 * MEM; e += blockDim.x) {
+        const int j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
+        float v[2][8];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int kq = c0 + kk + h, k = 2 * kq + p, n = m + k;
+          double2 z = cx(0, 0), d = cx(0, 0), f = cx(0, 0);
+          if (bl < nb && k < Km && kq < kp) {
+            const size_t off = static_cast<size_t>(b0 + bl) * 3 * G.S + G.moff[m] + k;
+            z = spec[off];
+            d = spec[off + G.S];
+            f = spec[off + 2 * G.S];
+            if (mixed && n == 0) f = cx(0, 0);  // phi_00 P_0^0 is added in FP64 below
+          }
+          const double kn = n >= 1 ? -a * a / (n * (n + 1.0)) : 0.0;  // psi, chi = kn (zeta, delta): sphere.cpp:185-190
+          v[h][0] = static_cast<float>(z.x);
+          v[h][1] = static_cast<float>(z.y);
+          v[h][2] = static_cast<float>(f.x);
+          v[h][3] = static_cast<float>(f.y);
+          v[h][4] = static_cast<float>(z.x * kn);
+          v[h][5] = static_cast<float>(z.y * kn);
+          v[h][6] = static_cast<float>(d.x * kn);
+          v[h][7] = static_cast<float>(d.y * kn);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          if (PREC == LMW_FP32) {
+            cf[(q * 32 + kk) * MEM + j] = v[0][q];
+            cf[(q * 32 + kk + 1) * MEM + j] = v[1][q];
+          } else {
+            __bf16 h0, l0, h1, l1;
+            split_bf16(v[0][q], h0, l0);
+            split_bf16(v[1][q], h1, l1);
+            typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<v2bf *>(ch + (q * MEM + j) * SP + kk) = v2bf{h0, h1};
+            *reinterpret_cast<v2bf *>(cl + (q * MEM + j) * SP + kk) = v2bf{l0, l1};
+          }
+        }
+      }
+      __syncthreads();
+      if (!active) continue;
+      const int jh = jt * 16 + (lane & 15);
+      if (PREC == LMW_FP32) {
+        const int rbase = G.poff[m] + (p ? kpe : 0);
+        const int steps = min(32, kp - c0) / 4;
+        for (int s = 0; s < steps; s++) {
+          const int kk = 4 * s + (lane >> 4);
+          const size_t ai = static_cast<size_t>(rbase + c0 + kk) * G.Jhp + jh;
+          const float aP = T.Ps32[ai], aH = T.Hs32[ai];
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const float bq = cf[(q * 32 + kk) * MEM + (lane & 15)];
+            accP[p][q] = mma32(aP, bq, accP[p][q]);
+            if (q >= 4) accH[p][q - 4] = mma32(aH, bq, accH[p][q - 4]);
+          }
+        }
+      } else {
+        const size_t base = static_cast<size_t>(T.soff[m]) + (p ? static_cast<size_t>(G.Jhp) * T.ke32[m] : 0) +
+                            static_cast<size_t>(jh) * kp32 + c0 + 8 * (lane >> 4);
+        const v8bf aPh = load8(T.Sph + base), aPl = load8(T.Spl + base), aHh = load8(T.Shh + base), aHl = load8(T.Shl + base);
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int bo = (q * MEM + (lane & 15)) * SP + 8 * (lane >> 4);
+          const v8bf bh = load8(ch + bo), bl = load8(cl + bo);
+          accP[p][q] = mma3(aPh, aPl, bh, bl, accP[p][q]);
+          if (q >= 4) accH[p][q - 4] = mma3(aHh, aHl, bh, bl, accH[p][q - 4]);
+        }
+      }
+    }
+  }
+  (void)nwave;
+  if (!active) return;
+  const int j = lane & 15, bl = mb + j;
+  if (bl >= nb) return;
+  const int b = b0 + bl;
+  double phi00 = 0;
+  if (mixed && m == 0) phi00 = spec[static_cast<size_t>(b) * 3 * G.S + 2 * G.S].x * 0.70710678118654752440;
+  const double dm = m;
+  for (int r = 0; r < 4; r++) {
+    const int jh = jt * 16 + 4 * (lane >> 4) + r;
+    if (jh >= G.Jh) continue;
+    for (int s = 0; s < 2; s++) {
+      const double sg = s == 0 ? 1.0 : -1.0;
+      const int jj = s == 0 ? G.Jh + jh : G.Jh - 1 - jh;
+      auto P = [&](int q) { return static_cast<double>(accP[0][q][r]) + sg * static_cast<double>(accP[1][q][r]); };
+      auto H = [&](int q) { return sg * static_cast<double>(accH[0][q][r]) + static_cast<double>(accH[1][q][r]); };
+      const double2 Z = cx(P(0), P(1)), F = cx(P(2) + phi00, P(3)), psP = cx(P(4), P(5)), chP = cx(P(6), P(7));
+      const double2 psH = cx(H(0), H(1)), chH = cx(H(2), H(3));
+      const double2 u = csub(cim(dm, chP), psH), v = cadd(cim(dm, psP), chH);  // sphere.cpp:200-201
+      float2 *o = Fs + ((static_cast<size_t>(b) * G.J + jj) * G.M1 + m) * 4;
+      o[0] = from_d2<float2>(cx(u.x / a, u.y / a));
+      o[1] = from_d2<float2>(cx(v.x / a, v.y / a));
+      o[2] = from_d2<float2>(Z);
+      o[3] = from_d2<float2>(F);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------- forward transform
+
+struct AnalTabs {
+  const float *A32;
+  const __bf16 *Abh, *Abl;
+};
+
+// Block: one (m, parity), 16 members, 4 waves over the 16-row tiles of that parity.  The K dimension runs over the
+// northern latitudes twice (P with X, then H with Y), staged in LDS in chunks of 64.  Quantities q = o1 re/im ..
+// o4 re/im (dzeta, ddelta, the Laplacian part of ddelta, dphi).
+template <int PREC>
+__global__ __launch_bounds__(256) void k_anal_mx(Geo G, AnalTabs T, const float2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
+                                                 double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
+                                                 double2 *__restrict__ raw, int b0, int nb, int mixed) {
+  constexpr int KC = 64, AP = 72;  // chunk of K; padded row pitch (bf16) of the BF16 chunk
+  extern __shared__ float lds_f[];
+  float *bf = lds_f;                                                         // FP32 chunk [q][k][member]
+  __bf16 *bh = reinterpret_cast<__bf16 *>(lds_f), *blo = bh + 8 * MEM * AP;  // BF16 chunk [q][member][k], hi and lo
+  const int m = blockIdx.x >> 1, p = blockIdx.x & 1, mb = blockIdx.y * MEM;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Rp = p ? G.kpo[m] : kpe, rbase = p ? kpe : 0;
+  if (Rp == 0) return;
+  const int nrt = Rp / 16, KK = 2 * G.Jhp;
+  v4f accv[2][8];
+  for (int t = 0; t < 2; t++)
+    for (int q = 0; q < 8; q++) accv[t][q] = v4f{0, 0, 0, 0};
+  for (int c0 = 0; c0 < KK; c0 += KC) {
+    __syncthreads();
+    // one thread per (member, pair of k): members fastest; a thread reads its two records of four contiguous
+    // complex operands and writes whole 32-bit LDS words
+    for (int e = threadIdx.x; e < (KC / 2) * MEM; e += blockDim.x) {
+      const int j = e % MEM, kk = 2 * (e / MEM);
+      const int bl = mb + j;
+      float vv[2][8];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int k = c0 + kk + h;
+        const int part = k >= G.Jhp ? 1 : 0, jh = part ? k - G.Jhp : k;
+        float2 g[4] = {make_float2(0, 0), make_float2(0, 0), make_float2(0, 0), make_float2(0, 0)};
+        if (bl < nb && jh < G.Jh) {
+          const float2 *src = Gb + (((static_cast<size_t>(b0 + bl) * G.M1 + m) * 2 + p) * 2 + part) * 4 * G.Jh + static_cast<size_t>(jh) * 4;
+#pragma unroll
+          for (int c = 0; c < 4; c++) g[c] = src[c];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) vv[h][q] = (q & 1) ? g[q >> 1].y : g[q >> 1].x;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if (PREC == LMW_FP32) {
+          bf[(q * KC + kk) * MEM + j] = vv[0][q];
+          bf[(q * KC + kk + 1) * MEM + j] = vv[1][q];
+        } else {
+          __bf16 h0, l0, h1, l1;
+          split_bf16(vv[0][q], h0, l0);
+          split_bf16(vv[1][q], h1, l1);
+          typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+          *reinterpret_cast<v2bf *>(bh + (q * MEM + j) * AP + kk) = v2bf{h0, h1};
+          *reinterpret_cast<v2bf *>(blo + (q * MEM + j) * AP + kk) = v2bf{l0, l1};
+        }
+      }
+    }
+    __syncthreads();
+    for (int ti = 0; ti < 2; ti++) {
+      const int t = wave + 4 * ti;
+      if (t >= nrt) break;
+      const int r = rbase + 16 * t + (lane & 15);
+      const int kc = min(KC, KK - c0);  // KK is a multiple of 32
+      if (PREC == LMW_FP32) {
+        for (int s = 0; s < kc / 4; s++) {
+          const int kk = 4 * s + (lane >> 4);
+          const float av = T.A32[static_cast<size_t>(KK) * G.poff[m] + static_cast<size_t>(c0 + kk) * kp + r];
+#pragma unroll
+          for (int q = 0; q < 8; q++) accv[ti][q] = mma32(av, bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
+        }
+      } else {
+        for (int s = 0; s < kc / 32; s++) {
+          const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + r) + c0 + 32 * s + 8 * (lane >> 4);
+          const v8bf ah = load8(T.Abh + ai), al = load8(T.Abl + ai);
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
+            accv[ti][q] = mma3(ah, al, load8(bh + bo), load8(blo + bo), accv[ti][q]);
+          }
+        }
+      }
+    }
+  }
+  const int j = lane & 15, bl = mb + j;
+  if (bl >= nb) return;
+  const int b = b0 + bl, Km = G.N - m + 1;
+  for (int ti = 0; ti < 2; ti++) {
+    const int t = wave + 4 * ti;
+    if (t >= nrt) break;
+    for (int r = 0; r < 4; r++) {
+      const int rl = 16 * t + 4 * (lane >> 4) + r;  // row within the parity block
+      const int k = 2 * rl + p;
+      if (k >= Km) continue;
+      const double2 o1 = cx(accv[ti][0][r], accv[ti][1][r]), o2 = cx(accv[ti][2][r], accv[ti][3][r]);
+      const double2 o3 = cx(accv[ti][4][r], accv[ti][5][r]), o4 = cx(accv[ti][6][r], accv[ti][7][r]);
+      anal_epilogue(G, mode, stage, dt, nu4, b, bl, m, k, o1, o2, o3, o4, x0, xs, acc, raw, mixed);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------- host side
+
+void SphereCore::build_mixed_tables() {
+  if (mixed_ready) return;
+  Geo G = geo();
+  std::vector<int> mrow(R), soff(M1), ke32(M1), ko32(M1);
+  int tot = 0;
+  for (int m = 0; m <= N; m++) {
+    for (int r = 0; r < h_kpe[m] + h_kpo[m]; r++) mrow[h_poff[m] + r] = m;
+    const int Km = N - m + 1;
+    ke32[m] = ((Km + 1) / 2 + 31) / 32 * 32;
+    ko32[m] = (Km / 2 + 31) / 32 * 32;
+    soff[m] = tot;
+    tot += Jhp * (ke32[m] + ko32[m]);
+  }
+  d_mrow.alloc(R);
+  d_mrow.upload(mrow.data(), R, 0, stream);
+  d_soff.alloc(M1);
+  d_soff.upload(soff.data(), M1, 0, stream);
+  d_ke32.alloc(M1);
+  d_ke32.upload(ke32.data(), M1, 0, stream);
+  d_ko32.alloc(M1);
+  d_ko32.upload(ko32.data(), M1, 0, stream);
+  const size_t na = static_cast<size_t>(2 * Jhp) * R;
+  A32.alloc(na);
+  Abh.alloc(na);
+  Abl.alloc(na);
+  Ps32.alloc(static_cast<size_t>(R) * Jhp);
+  Hs32.alloc(static_cast<size_t>(R) * Jhp);
+  Fs32.alloc(Fs.n);
+  Gb32.alloc(Gb.n);
+  Sph.alloc(tot);
+  Spl.alloc(tot);
+  Shh.alloc(tot);
+  Shl.alloc(tot);
+  hipLaunchKernelGGL(k_build_anal_tables, dim3(1024), dim3(256), 0, stream, G, R, Pa.p, Ha.p, A32.p, Abh.p, Abl.p, d_mrow.p);
+  LMW_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_build_synth_tables, dim3(M1), dim3(256), 0, stream, G, R, Ps.p, Hs.p, Ps32.p, Hs32.p, d_soff.p, d_ke32.p, d_ko32.p,
+                     Sph.p, Spl.p, Shh.p, Shl.p);
+  LMW_LAUNCH_CHECK();
+  sync();
+  mixed_ready = true;
+}
+
+void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
+  SynthTabs T{Ps32.p, Hs32.p, Sph.p, Spl.p, Shh.p, Shl.p, d_soff.p, d_ke32.p, d_ko32.p};
+  const int waves = (Jh + 15) / 16;
+  dim3 grd(M1, (nb + MEM - 1) / MEM), blk(64 * waves);
+  if (precision == LMW_FP32)
+    hipLaunchKernelGGL(k_synth_mx<LMW_FP32>, grd, blk, 8 * 32 * MEM * sizeof(float), stream, geo(), T, spec, Fs32.p, b0, nb, 1);
+  else
+    hipLaunchKernelGGL(k_synth_mx<LMW_BF16X3>, grd, blk, 2 * 8 * 40 * MEM * sizeof(__bf16), stream, geo(), T, spec, Fs32.p, b0, nb, 1);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  AnalTabs T{A32.p, Abh.p, Abl.p};
+  dim3 grd(2 * M1, (nb + MEM - 1) / MEM), blk(256);
+  const size_t lds = precision == LMW_FP32 ? 8 * 64 * MEM * sizeof(float) : 2 * 8 * 72 * MEM * sizeof(__bf16);
+  if (precision == LMW_FP32)
+    hipLaunchKernelGGL(k_anal_mx<LMW_FP32>, grd, blk, lds, stream, geo(), T, Gb32.p, mode, stage, dt, prm.diffusion, x0.p, xs.p, acc.p, raw_out, b0,
+                       nb, 1);
+  else
+    hipLaunchKernelGGL(k_anal_mx<LMW_BF16X3>, grd, blk, lds, stream, geo(), T, Gb32.p, mode, stage, dt, prm.diffusion, x0.p, xs.p, acc.p, raw_out,
+                       b0, nb, 1);
+  LMW_LAUNCH_CHECK();
+}
+
+}  // namespace lmw

@@ -45,9 +45,12 @@ struct SphereCore {
   DevArray<int> moff, poff, kpe, kpo;
   DevArray<double> mu, w;
   DevArray<double> Ps, Hs, Pa, Ha;             // FP64 tables
-  DevArray<float> Ps32, Hs32, Pa32, Ha32;      // FP32 tables (LMW_FP32)
-  DevArray<uint32_t> Psb, Hsb, Pab, Hab;       // split BF16 tables, hi|lo packed per entry (LMW_BF16X3)
-  DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K/2
+  // mixed-precision operands of the Legendre kernels, rounded/split once from the FP64 tables (precision.hip)
+  bool mixed_ready = false;
+  DevArray<int> d_mrow, d_soff, d_ke32, d_ko32;
+  DevArray<float> A32, Ps32, Hs32;
+  DevArray<__bf16> Abh, Abl, Sph, Spl, Shh, Shl;
+  DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K
   DevArray<double2> x0, xs, acc;               // [b][3][S]
   DevArray<double2> Fs;                        // [b][J][M1][4]
   DevArray<double2> Gb;                        // [b][M1][Jh][16]
@@ -56,6 +59,8 @@ struct SphereCore {
   DevArray<double> in3;                        // [b][3][J][K] input grids for the analysis path
   DevArray<double2> spec_io;                   // [b][4][S] raw transform outputs / inputs
   DevArray<double> red;                        // [b][16] reductions
+  DevArray<int> mflags;                        // [b] bit 0 untilted axis, bit 1 no orography
+  DevArray<float2> Fs32, Gb32;                 // FP32 storage of Fs and Gb for the mixed-precision step
   hipGraphExec_t graph = nullptr;
   double graph_dt = 0;
 
@@ -70,6 +75,7 @@ struct SphereCore {
 
   // set-up (member b)
   void set_axis_tilt(int b, double alpha);
+  void update_flags(int b);
   void set_orography(int b, const double *hs_host);
   void set_state_grid(int b, const double *h, const double *u, const double *v);
   void set_spectral_packed(int b, int field, const double2 *packed);  // host packed triangle
@@ -77,6 +83,7 @@ struct SphereCore {
   void mark_reference(int b);
 
   // stepping (all members)
+  void launch_stage(int stage, double dt);
   void enqueue_step(double dt);
   void steps(double dt, int64_t count);
   void steps_graph(double dt, int64_t count);
@@ -98,10 +105,13 @@ struct SphereCore {
 
   // launches
   void launch_synth(const double2 *spec, int b0, int nb);
-  void launch_tend_grid(int b0, int nb);
+  void launch_tend_grid(int b0, int nb, int mixed = 0);
   void launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
   void launch_prep(int mode, int b0, int nb);
   void launch_to_grid(int mode, double *out, int b0, int nb);
+  void build_mixed_tables();
+  void launch_synth_mx(const double2 *spec, int b0, int nb);
+  void launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
 
   size_t packed_index(int m, int n) const { return static_cast<size_t>(h_moff[m] + (n - m)); }
 };

@@ -14,7 +14,9 @@
 // FP64 throughout.  The FFT twiddles are exp(-2 pi i q / K) rounded once on the host.
 #include <cmath>
 
+#include "fft_lds.h"
 #include "sphere_core.h"
+#include "sphere_epilogue.h"
 
 namespace lmw {
 
@@ -55,8 +57,8 @@ __device__ void fft_lds(double2 *buf, int nf, int L, int logL, const double2 *tw
 // reference's separate transform of each row does, sphere.cpp:123-136).  tw[q] = exp(-2 pi i q / K), q < L.
 //
 // Forward: arr = FFT_L(x[2k] + i x[2k+1]); returns X_m / K for 0 <= m < L (fourier_forward: a[m] / nlon).
-__device__ inline double2 rfft_coef(const double2 *arr, int m, int L, const double2 *tw, double invK) {
-  const double2 Zm = arr[m], Zc = arr[(L - m) & (L - 1)];
+template <class F> __device__ inline double2 rfft_coef(const double2 *arr, int m, const double2 *tw, double invK) {
+  const double2 Zm = arr[F::phys(F::pos(m))], Zc = arr[F::phys(F::pos((F::L - m) & (F::L - 1)))];
   const double2 Fe = cx(0.5 * (Zm.x + Zc.x), 0.5 * (Zm.y - Zc.y));  // (Zm + conj Zc) / 2: even samples
   const double2 Fo = cx(0.5 * (Zm.y + Zc.y), 0.5 * (Zc.x - Zm.x));  // (Zm - conj Zc) / 2i: odd samples
   return cscale(cadd(Fe, cmul(tw[m], Fo)), invK);
@@ -65,23 +67,30 @@ __device__ inline double2 rfft_coef(const double2 *arr, int m, int L, const doub
 // Backward: entry m' (0 <= m' < L) of the half-length spectrum whose backward transform is x[2k] + i x[2k+1] for
 // the real row x_k = sum_m a_m exp(2 pi i m k / K) with a_0 = Re F_0, a_m = F_m, a_{K-m} = conj F_m (m = 1..N) and
 // zero elsewhere (fourier_backward, sphere.cpp:130-136).  F_m is read at src[4 m].
-__device__ inline double2 irfft_entry(const double2 *__restrict__ src, int mp, int L, int N, const double2 *tw) {
+template <class TS> __device__ inline double2 irfft_entry(const TS *__restrict__ src, int mp, int L, int N, const double2 *tw) {
   double2 lo = cx(0, 0), hi = cx(0, 0);  // a_{m'} and a_{m'+L}
-  if (mp == 0) lo = cx(src[0].x, 0.0);
-  else if (mp <= N) lo = src[4 * mp];
-  if (mp >= 1 && L - mp <= N) hi = cconj(src[4 * (L - mp)]);
+  if (mp == 0) lo = cx(to_d2(src[0]).x, 0.0);
+  else if (mp <= N) lo = to_d2(src[4 * mp]);
+  if (mp >= 1 && L - mp <= N) hi = cconj(to_d2(src[4 * (L - mp)]));
   const double2 E = cadd(lo, hi), O = cmul(csub(lo, hi), cconj(tw[mp]));
   return cx(E.x - O.y, E.y + O.x);  // E + i O
 }
 
-// Writes the 16 analysis operands of one (m, latitude pair) from the per-latitude values (north n, south s).
-__device__ inline void store_operands(double2 *o, const double2 Xn[4], const double2 Xs[4], const double2 Yn[4], const double2 Ys[4]) {
+// Writes the analysis operands of one (member, m, latitude pair) from the per-latitude values (north n, south s).
+// Layout per (b, m): [parity p][part: 0 = P operand X, 1 = H operand Y][jh][4], where even rows (p = 0) use the
+// north/south sums X+ = Xn + Xs and differences Y- = Yn - Ys, odd rows X- and Y+ (P(-mu) = (-1)^(n-m) P(mu),
+// H(-mu) = -(-1)^(n-m) H(mu)).
+template <class TS>
+__device__ inline void store_operands(TS *Gb, size_t bm, int Jh, int jh, const double2 Xn[4], const double2 Xs[4], const double2 Yn[4],
+                                      const double2 Ys[4]) {
+  TS *o = Gb + bm * 16 * Jh + static_cast<size_t>(jh) * 4;
+  const size_t part = static_cast<size_t>(Jh) * 4;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    o[i] = cadd(Xn[i], Xs[i]);
-    o[4 + i] = csub(Xn[i], Xs[i]);
-    o[8 + i] = cadd(Yn[i], Ys[i]);
-    o[12 + i] = csub(Yn[i], Ys[i]);
+    o[i] = from_d2<TS>(cadd(Xn[i], Xs[i]));             // p 0, X
+    o[part + i] = from_d2<TS>(csub(Yn[i], Ys[i]));      // p 0, Y
+    o[2 * part + i] = from_d2<TS>(csub(Xn[i], Xs[i]));  // p 1, X
+    o[3 * part + i] = from_d2<TS>(cadd(Yn[i], Ys[i]));  // p 1, Y
   }
 }
 }  // namespace
@@ -224,45 +233,58 @@ __global__ __launch_bounds__(256) void k_synth(Geo G, const double *__restrict__
 
 // Grid-point work of one latitude pair: inverse FFTs, nonlinear products, forward FFTs, analysis operands.
 // Rows in LDS: inverse U, V, zeta, phi and forward A, B, U phi, V phi, phi + g hs + E, each at (north, south).
-__global__ __launch_bounds__(256) void k_tend_grid(Geo G, const double2 *__restrict__ Fs, const double *__restrict__ fgrid,
+// Storage TS of the Fourier coefficients and analysis operands: double2 (FP64 path) or float2 (mixed precision).
+// mflags[b]: bit 0 = rotation axis untilted (f = 2 Omega mu computed, not read), bit 1 = no orography (hs = 0).
+template <int R1, int R2, class TS>
+__global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__ Fs, const double *__restrict__ fgrid,
                                                    const double *__restrict__ hs, const double2 *__restrict__ twg,
                                                    const double *__restrict__ mu, const double *__restrict__ w,
-                                                   double2 *__restrict__ Gb, int b0) {
+                                                   TS *__restrict__ Gb, int b0, int mixed, const double2 *__restrict__ xs,
+                                                   const int *__restrict__ mflags) {
+  using F = Fft2<R1, R2>;
+  constexpr int L = F::L, P = F::P;
   extern __shared__ double2 lds[];
-  const int K = G.K, L = K >> 1, logL = G.logK - 1, M1 = G.M1, tid = threadIdx.x;
+  const int K = 2 * L, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
-  double2 *buf = lds, *tw = lds + 10 * L;
-  for (int i = tid; i < L; i += blockDim.x) tw[i] = twg[i];
+  double2 *buf = lds, *tw = lds + 10 * P;
+  // mixed precision: the phi fluxes and the Bernoulli function are formed with phi - phibar (the area mean
+  // phi_00 / sqrt 2), whose exact contribution -phibar delta the forward transform's epilogue adds back
+  const double phibar = mixed ? xs[static_cast<size_t>(b) * 3 * G.S + 2 * G.S].x * 0.70710678118654752440 : 0.0;
+  for (int i = tid; i < K; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
   for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
-    const int r = idx >> logL, mp = idx & (L - 1);
+    const int r = idx / L, mp = idx - r * L;
     const int j = (r & 1) ? js : jn;
-    const double2 *src = Fs + (static_cast<size_t>(b) * G.J + j) * M1 * 4 + (r >> 1);
-    buf[r * L + brev(mp, logL)] = irfft_entry(src, mp, L, G.N, tw);
+    const TS *src = Fs + (static_cast<size_t>(b) * G.J + j) * M1 * 4 + (r >> 1);
+    buf[r * P + F::phys(mp)] = irfft_entry(src, mp, L, G.N, tw);
   }
   __syncthreads();
-  fft_lds(buf, 8, L, logL, tw, 1, +1);
+  fft2<R1, R2, +1>(buf, 8, tw);
   const double om = 1 - mu[jn] * mu[jn];  // identical for the southern row: mu(s) = -mu(n)
   const double g = G.g;
+  const int flags = mflags[b];
+  const bool untilted = flags & 1, flat = flags & 2;
   double2 res[10];
   const int k = tid;  // one pair of longitudes (2k, 2k+1) per thread: L <= 256
   if (k < L) {
+    const int pk = F::phys(F::pos(k));
 #pragma unroll
     for (int s = 0; s < 2; s++) {
       const int j = s ? js : jn;
-      const double2 U = buf[(0 + s) * L + k], V = buf[(2 + s) * L + k], Z = buf[(4 + s) * L + k], P = buf[(6 + s) * L + k];
+      const double2 U = buf[(0 + s) * P + pk], V = buf[(2 + s) * P + pk], Z = buf[(4 + s) * P + pk], Ph = buf[(6 + s) * P + pk];
       const size_t gi = (static_cast<size_t>(b) * G.J + j) * K + 2 * k;
       double o[5][2];
 #pragma unroll
       for (int e = 0; e < 2; e++) {
-        const double u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = e ? P.y : P.x;
-        const double eta = z + fgrid[gi + e];
+        const double u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = (e ? Ph.y : Ph.x) - phibar;
+        // untilted f = 2 Omega (-cos(lon) cos(lat) 0 + mu 1) = (2 Omega) mu exactly (sphere.cpp:116)
+        const double eta = z + (untilted ? (2 * G.omega) * mu[j] : fgrid[gi + e]);
         o[0][e] = u * eta;
         o[1][e] = v * eta;
         o[2][e] = u * ph;
         o[3][e] = v * ph;
-        o[4][e] = ph + g * hs[gi + e] + (u * u + v * v) / (2 * om);  // Bernoulli: g (h + hs) + E (sphere.cpp:357)
+        o[4][e] = ph + g * (flat ? 0.0 : hs[gi + e]) + (u * u + v * v) / (2 * om);  // Bernoulli: g (h + hs) + E (sphere.cpp:357)
       }
 #pragma unroll
       for (int q = 0; q < 5; q++) res[2 * q + s] = cx(o[q][0], o[q][1]);
@@ -270,17 +292,17 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const double2 *__restr
   }
   __syncthreads();
   if (k < L) {
-    const int q = brev(k, logL);
+    const int pk = F::phys(k);
 #pragma unroll
-    for (int r = 0; r < 10; r++) buf[r * L + q] = res[r];
+    for (int r = 0; r < 10; r++) buf[r * P + pk] = res[r];
   }
   __syncthreads();
-  fft_lds(buf, 10, L, logL, tw, 1, -1);
+  fft2<R1, R2, -1>(buf, 10, tw);
   const double a = G.a, wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / a, invK = 1.0 / K;
   for (int m = tid; m < M1; m += blockDim.x) {
     double2 c[10];
 #pragma unroll
-    for (int r = 0; r < 10; r++) c[r] = rfft_coef(buf + r * L, m, L, tw, invK);
+    for (int r = 0; r < 10; r++) c[r] = rfft_coef<F>(buf + r * P, m, tw, invK);
     const double dm = m;
     // dzeta -= (i m A wd) P - (B wd) H; ddelta += (i m B wd) P + (A wd) H + (E w) P n(n+1)/a^2;
     // dphi -= (i m U phi wd) P - (V phi wd) H   (sphere.cpp:374-383); index 0 north, 1 south
@@ -297,26 +319,29 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const double2 *__restr
       X[s][3] = cscale(cim(dm, UP), -wd);
       Y[s][3] = cscale(VP, wd);
     }
-    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, X[0], X[1], Y[0], Y[1]);
+    store_operands(Gb, static_cast<size_t>(b) * M1 + m, G.Jh, jh, X[0], X[1], Y[0], Y[1]);
   }
 }
 
 // Grid inputs to analysis operands: the set_state_grid analyses (sphere.cpp:217-231), analyze_divergence
 // (:164-177) and analyze_scalar (:138-149).  Rows: STATE U, V, phi; DIV A, B; SCALAR the field; each (north, south).
+template <int R1, int R2>
 __global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__restrict__ in3, const double2 *__restrict__ twg,
                                               const double *__restrict__ mu, const double *__restrict__ w, double2 *__restrict__ Gb, int b0) {
+  using F = Fft2<R1, R2>;
+  constexpr int L = F::L, P = F::P;
   extern __shared__ double2 lds[];
-  const int K = G.K, L = K >> 1, logL = G.logK - 1, M1 = G.M1, tid = threadIdx.x;
+  const int K = 2 * L, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
-  double2 *buf = lds, *tw = lds + 10 * L;
+  double2 *buf = lds, *tw = lds + 10 * P;
   const size_t plane = static_cast<size_t>(G.J) * K;
   const double *g0 = in3 + static_cast<size_t>(blockIdx.y) * 3 * plane;
-  for (int i = tid; i < L; i += blockDim.x) tw[i] = twg[i];
+  for (int i = tid; i < K; i += blockDim.x) tw[i] = twg[i];
   const double c = sqrt(1 - mu[jn] * mu[jn]);
   const int nrows = mode == PREP_STATE ? 6 : mode == PREP_DIV ? 4 : 2;
   for (int idx = tid; idx < nrows * L; idx += blockDim.x) {
-    const int r = idx >> logL, k = idx & (L - 1);
+    const int r = idx / L, k = idx - r * L;
     const int field = r >> 1;
     const size_t o = static_cast<size_t>((r & 1) ? js : jn) * K + 2 * k;
     double x0, x1;
@@ -330,10 +355,10 @@ __global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__r
       x0 = src[o];
       x1 = src[o + 1];
     }
-    buf[r * L + brev(k, logL)] = cx(x0, x1);
+    buf[r * P + F::phys(k)] = cx(x0, x1);
   }
   __syncthreads();
-  fft_lds(buf, nrows, L, logL, tw, 1, -1);
+  fft2<R1, R2, -1>(buf, nrows, tw);
   const double wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / G.a, invK = 1.0 / K;
   for (int m = tid; m < M1; m += blockDim.x) {
     const double dm = m;
@@ -342,23 +367,23 @@ __global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__r
       for (int i = 0; i < 4; i++) X[s][i] = Y[s][i] = cx(0, 0);
     for (int s = 0; s < 2; s++) {
       if (mode == PREP_STATE) {
-        const double2 U = rfft_coef(buf + (0 + s) * L, m, L, tw, invK), V = rfft_coef(buf + (2 + s) * L, m, L, tw, invK);
-        const double2 P = rfft_coef(buf + (4 + s) * L, m, L, tw, invK);
+        const double2 U = rfft_coef<F>(buf + (0 + s) * P, m, tw, invK), V = rfft_coef<F>(buf + (2 + s) * P, m, tw, invK);
+        const double2 Ph = rfft_coef<F>(buf + (4 + s) * P, m, tw, invK);
         // zeta = D(V, -U), delta = D(U, V) (sphere.cpp:211-212), phi by analyze_scalar
         X[s][0] = cscale(cim(dm, V), wd);
         Y[s][0] = cscale(U, wd);
         X[s][1] = cscale(cim(dm, U), wd);
         Y[s][1] = cscale(V, -wd);
-        X[s][3] = cscale(P, wj);
+        X[s][3] = cscale(Ph, wj);
       } else if (mode == PREP_DIV) {
-        const double2 A = rfft_coef(buf + (0 + s) * L, m, L, tw, invK), B = rfft_coef(buf + (2 + s) * L, m, L, tw, invK);
+        const double2 A = rfft_coef<F>(buf + (0 + s) * P, m, tw, invK), B = rfft_coef<F>(buf + (2 + s) * P, m, tw, invK);
         X[s][0] = cscale(cim(dm, A), wd);
         Y[s][0] = cscale(B, -wd);
       } else {
-        X[s][3] = cscale(rfft_coef(buf + s * L, m, L, tw, invK), wj);
+        X[s][3] = cscale(rfft_coef<F>(buf + s * P, m, tw, invK), wj);
       }
     }
-    store_operands(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 16, X[0], X[1], Y[0], Y[1]);
+    store_operands(Gb, static_cast<size_t>(b) * M1 + m, G.Jh, jh, X[0], X[1], Y[0], Y[1]);
   }
 }
 
@@ -384,8 +409,13 @@ __global__ __launch_bounds__(256) void k_anal(Geo G, const double *__restrict__ 
     const int cnt = min(CH, G.Jh - j0);
     __syncthreads();
     if (bl < nb) {
-      const double2 *src = Gb + ((static_cast<size_t>(b) * G.M1 + m) * G.Jh + j0) * 16;
-      for (int i = tx; i < cnt * 16; i += 64) sg[ty][i >> 4][i & 15] = src[i];
+      // slots in sg: X+ 0..3, X- 4..7, Y+ 8..11, Y- 12..15, from the blocks (p, part) = (0,X) (0,Y) (1,X) (1,Y)
+      const double2 *src = Gb + (static_cast<size_t>(b) * G.M1 + m) * 16 * G.Jh;
+      for (int i = tx; i < 16 * cnt; i += 64) {
+        const int blk = i / (4 * cnt), rem = i - blk * 4 * cnt, jj = rem >> 2, c = rem & 3;
+        const int slot = (blk == 0 ? 0 : blk == 1 ? 12 : blk == 2 ? 4 : 8) + c;
+        sg[ty][jj][slot] = src[(static_cast<size_t>(blk) * G.Jh + j0 + jj) * 4 + c];
+      }
     }
     __syncthreads();
     if (valid) {
@@ -403,97 +433,42 @@ __global__ __launch_bounds__(256) void k_anal(Geo G, const double *__restrict__ 
     }
   }
   if (!valid) return;
-  const int n = m + k;
-  const size_t q = static_cast<size_t>(G.moff[m]) + k;
-  const size_t S = G.S;
-  if (mode == ANAL_RAW) {  // raw outputs are indexed by the local member
-    raw[(static_cast<size_t>(bl) * 4 + 0) * S + q] = o1;
-    raw[(static_cast<size_t>(bl) * 4 + 1) * S + q] = o2;
-    raw[(static_cast<size_t>(bl) * 4 + 2) * S + q] = o3;
-    raw[(static_cast<size_t>(bl) * 4 + 3) * S + q] = o4;
-    return;
-  }
-  if (mode == ANAL_STATE) {
-    if (q == 0) o1 = o2 = cx(0, 0);  // the constant is not defined by the winds (sphere.cpp:213-214)
-    const size_t i0 = static_cast<size_t>(b) * 3 * S + q;
-    x0[i0] = o1;
-    x0[i0 + S] = o2;
-    x0[i0 + 2 * S] = o4;
-    xs[i0] = o1;
-    xs[i0 + S] = o2;
-    xs[i0 + 2 * S] = o4;
-    return;
-  }
-  double2 kz = o1, kd = cadd(o2, cscale(o3, n * (n + 1.0) / (G.a * G.a))), kf = o4;
-  if (q == 0) kz = kd = kf = cx(0, 0);  // sphere.cpp:387-389
-  if (mode == ANAL_TENDOUT) {  // tendencies only, indexed by the local member
-    raw[(static_cast<size_t>(bl) * 4 + 0) * S + q] = kz;
-    raw[(static_cast<size_t>(bl) * 4 + 1) * S + q] = kd;
-    raw[(static_cast<size_t>(bl) * 4 + 2) * S + q] = kf;
-    return;
-  }
-  {
-#pragma clang fp contract(off)
-    const double2 kk[3] = {kz, kd, kf};
-    double fv = 1.0, fpp = 1.0;
-    if (stage == 4 && nu4 > 0 && n >= 1) {  // del^4 filter (sphere.cpp:392-405)
-      const double a2 = G.a * G.a, l = n * (n + 1.0);
-      const double kv = (l - 2) / a2, kq = l / a2;
-      fv = exp(-nu4 * kv * kv * dt);
-      fpp = exp(-nu4 * kq * kq * dt);
-    }
-    for (int fld = 0; fld < 3; fld++) {
-      const size_t i = (static_cast<size_t>(b) * 3 + fld) * S + q;
-      const double2 s0 = x0[i], kv = kk[fld];
-      if (stage == 1) {
-        acc[i] = kv;
-        xs[i] = cadd(s0, cscale(kv, 0.5 * dt));
-      } else if (stage == 2) {
-        acc[i] = cadd(acc[i], cscale(kv, 2.0));
-        xs[i] = cadd(s0, cscale(kv, 0.5 * dt));
-      } else if (stage == 3) {
-        acc[i] = cadd(acc[i], cscale(kv, 2.0));
-        xs[i] = cadd(s0, cscale(kv, dt));
-      } else {
-        double2 nx = cadd(s0, cscale(cadd(acc[i], kv), dt / 6));
-        if (stage == 4 && nu4 > 0 && n >= 1) nx = cscale(nx, fld == 2 ? fpp : fv);
-        x0[i] = nx;
-        xs[i] = nx;
-      }
-    }
-  }
+  anal_epilogue(G, mode, stage, dt, nu4, b, bl, m, k, o1, o2, o3, o4, x0, xs, acc, raw, 0);
 }
 
 // Inverse FFTs of the four Fourier fields to the grid: diagnostics (h = phi/g, u = U/cos, v = V/cos, zeta;
 // sphere.cpp:486-508) or the raw rows U, V, zeta, phi.
+template <int R1, int R2>
 __global__ __launch_bounds__(256) void k_to_grid(Geo G, int mode, const double2 *__restrict__ Fs, const double2 *__restrict__ twg,
                                                  const double *__restrict__ mu, double *__restrict__ out, int b0) {
+  using F = Fft2<R1, R2>;
+  constexpr int L = F::L, P = F::P;
   extern __shared__ double2 lds[];
-  const int K = G.K, L = K >> 1, logL = G.logK - 1, M1 = G.M1, tid = threadIdx.x;
+  const int K = 2 * L, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
-  double2 *buf = lds, *tw = lds + 10 * L;
-  for (int i = tid; i < L; i += blockDim.x) tw[i] = twg[i];
+  double2 *buf = lds, *tw = lds + 10 * P;
+  for (int i = tid; i < K; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
   for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
-    const int r = idx >> logL, mp = idx & (L - 1);
+    const int r = idx / L, mp = idx - r * L;
     const int j = (r & 1) ? js : jn;
     const double2 *src = Fs + (static_cast<size_t>(b) * G.J + j) * M1 * 4 + (r >> 1);
-    buf[r * L + brev(mp, logL)] = irfft_entry(src, mp, L, G.N, tw);
+    buf[r * P + F::phys(mp)] = irfft_entry(src, mp, L, G.N, tw);
   }
   __syncthreads();
-  fft_lds(buf, 8, L, logL, tw, 1, +1);
+  fft2<R1, R2, +1>(buf, 8, tw);
   const double c = sqrt(1 - mu[jn] * mu[jn]);
   const size_t plane = static_cast<size_t>(G.J) * K;
   double *o = out + static_cast<size_t>(b) * 4 * plane;
   for (int idx = tid; idx < 2 * L; idx += blockDim.x) {
-    const int s = idx >> logL, k = idx & (L - 1);
-    const int j = s ? js : jn;
-    const double2 U = buf[(0 + s) * L + k], V = buf[(2 + s) * L + k], Z = buf[(4 + s) * L + k], P = buf[(6 + s) * L + k];
+    const int s = idx / L, k = idx - s * L;
+    const int j = s ? js : jn, pk = F::phys(F::pos(k));
+    const double2 U = buf[(0 + s) * P + pk], V = buf[(2 + s) * P + pk], Z = buf[(4 + s) * P + pk], Ph = buf[(6 + s) * P + pk];
     const size_t i = static_cast<size_t>(j) * K + 2 * k;
     if (mode == GRID_DIAG) {
-      o[i] = P.x / G.g;
-      o[i + 1] = P.y / G.g;
+      o[i] = Ph.x / G.g;
+      o[i + 1] = Ph.y / G.g;
       o[plane + i] = U.x / c;
       o[plane + i + 1] = U.y / c;
       o[2 * plane + i] = V.x / c;
@@ -507,8 +482,8 @@ __global__ __launch_bounds__(256) void k_to_grid(Geo G, int mode, const double2 
       o[plane + i + 1] = V.y;
       o[2 * plane + i] = Z.x;
       o[2 * plane + i + 1] = Z.y;
-      o[3 * plane + i] = P.x;
-      o[3 * plane + i + 1] = P.y;
+      o[3 * plane + i] = Ph.x;
+      o[3 * plane + i + 1] = Ph.y;
     }
   }
 }
@@ -644,13 +619,13 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
   Hs.alloc(tab);
   Pa.alloc(tab);
   Ha.alloc(tab);
-  std::vector<double2> htw(K / 2);
-  for (int q = 0; q < K / 2; q++) {
+  std::vector<double2> htw(K);
+  for (int q = 0; q < K; q++) {
     const long double ang = 2.0L * 3.141592653589793238462643383279502884L * q / K;
     htw[q] = make_double2(static_cast<double>(cosl(ang)), static_cast<double>(-sinl(ang)));
   }
-  tw.alloc(K / 2);
-  tw.upload(htw.data(), K / 2, 0, stream);
+  tw.alloc(K);
+  tw.upload(htw.data(), K, 0, stream);
   const size_t nstate = static_cast<size_t>(B) * 3 * S;
   x0.alloc(nstate);
   xs.alloc(nstate);
@@ -668,6 +643,7 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
   t.assign(B, 0.0);
   tilt.assign(B, 0.0);
   has_orography.assign(B, 0);
+  mflags.alloc(B);
   Geo G = geo();
   hipLaunchKernelGGL(k_gauss, dim3(1), dim3(256), 0, stream, J, mu.p, w.p);
   LMW_LAUNCH_CHECK();
@@ -707,8 +683,15 @@ Geo SphereCore::geo() const {
 
 void SphereCore::sync() { LMW_HIP(hipStreamSynchronize(stream)); }
 
+void SphereCore::update_flags(int b) {
+  const int fl = (tilt[b] == 0.0 ? 1 : 0) | (has_orography[b] ? 0 : 2);
+  mflags.upload(&fl, 1, b, stream);
+  sync();  // the source is a stack variable
+}
+
 void SphereCore::set_axis_tilt(int b, double alpha) {
   tilt[b] = alpha;
+  update_flags(b);
   const size_t plane = static_cast<size_t>(J) * K;
   hipLaunchKernelGGL(k_coriolis, dim3(J), dim3(256), 0, stream, geo(), mu.p, f.p + b * plane, alpha);
   LMW_LAUNCH_CHECK();
@@ -718,6 +701,7 @@ void SphereCore::set_orography(int b, const double *hsh) {
   const size_t plane = static_cast<size_t>(J) * K;
   hs.upload(hsh, plane, b * plane, stream);
   has_orography[b] = 1;
+  update_flags(b);
 }
 
 void SphereCore::launch_synth(const double2 *spec, int b0, int nb) {
@@ -726,10 +710,43 @@ void SphereCore::launch_synth(const double2 *spec, int b0, int nb) {
   LMW_LAUNCH_CHECK();
 }
 
-static size_t grid_lds_bytes(int K) { return 11 * static_cast<size_t>(K / 2) * sizeof(double2); }  // 10 rows + twiddles
+// LDS of the grid kernels: 10 padded rows of K/2 plus the K twiddles
+static size_t grid_lds_bytes(int K) {
+  const int L = K / 2, R2 = L >= 128 ? (L == 256 ? 16 : 8) : L >= 32 ? (L == 64 ? 8 : 4) : L == 16 ? 4 : 2;
+  return (10 * static_cast<size_t>(L + L / R2) + K) * sizeof(double2);
+}
 
-void SphereCore::launch_tend_grid(int b0, int nb) {
-  hipLaunchKernelGGL(k_tend_grid, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs.p, f.p, hs.p, tw.p, mu.p, w.p, Gb.p, b0);
+// L = K/2 -> (R1, R2): 256 (16,16), 128 (16,8), 64 (8,8), 32 (8,4), 16 (4,4), 8 (4,2)
+#define LMW_DISPATCH_L(L, KERNEL, ...)                                               \
+  switch (L) {                                                                       \
+    case 256: hipLaunchKernelGGL((KERNEL<16, 16>), __VA_ARGS__); break;             \
+    case 128: hipLaunchKernelGGL((KERNEL<16, 8>), __VA_ARGS__); break;              \
+    case 64: hipLaunchKernelGGL((KERNEL<8, 8>), __VA_ARGS__); break;                \
+    case 32: hipLaunchKernelGGL((KERNEL<8, 4>), __VA_ARGS__); break;                \
+    case 16: hipLaunchKernelGGL((KERNEL<4, 4>), __VA_ARGS__); break;                \
+    case 8: hipLaunchKernelGGL((KERNEL<4, 2>), __VA_ARGS__); break;                 \
+    default: invalid("unsupported grid size");                                     \
+  }
+// the same with a storage type as third template argument
+#define LMW_DISPATCH_LT(L, KERNEL, TS, ...)                                          \
+  switch (L) {                                                                       \
+    case 256: hipLaunchKernelGGL((KERNEL<16, 16, TS>), __VA_ARGS__); break;         \
+    case 128: hipLaunchKernelGGL((KERNEL<16, 8, TS>), __VA_ARGS__); break;          \
+    case 64: hipLaunchKernelGGL((KERNEL<8, 8, TS>), __VA_ARGS__); break;            \
+    case 32: hipLaunchKernelGGL((KERNEL<8, 4, TS>), __VA_ARGS__); break;            \
+    case 16: hipLaunchKernelGGL((KERNEL<4, 4, TS>), __VA_ARGS__); break;            \
+    case 8: hipLaunchKernelGGL((KERNEL<4, 2, TS>), __VA_ARGS__); break;             \
+    default: invalid("unsupported grid size");                                     \
+  }
+
+void SphereCore::launch_tend_grid(int b0, int nb, int mixed) {
+  if (mixed) {
+    LMW_DISPATCH_LT(K / 2, k_tend_grid, float2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs32.p, f.p, hs.p, tw.p, mu.p, w.p,
+                    Gb32.p, b0, mixed, xs.p, mflags.p)
+  } else {
+    LMW_DISPATCH_LT(K / 2, k_tend_grid, double2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs.p, f.p, hs.p, tw.p, mu.p, w.p,
+                    Gb.p, b0, mixed, xs.p, mflags.p)
+  }
   LMW_LAUNCH_CHECK();
 }
 
@@ -741,18 +758,23 @@ void SphereCore::launch_anal(int mode, int stage, double dt, int b0, int nb, dou
 
 void SphereCore::launch_prep(int mode, int b0, int nb) {
   // in3 holds the input grids of members b0.. at offset 0 (k_prep indexes it by the local member)
-  hipLaunchKernelGGL(k_prep, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), mode, in3.p, tw.p, mu.p, w.p, Gb.p, b0);
+  LMW_DISPATCH_L(K / 2, k_prep, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), mode, in3.p, tw.p, mu.p, w.p, Gb.p, b0);
   LMW_LAUNCH_CHECK();
 }
 
 void SphereCore::launch_to_grid(int mode, double *out, int b0, int nb) {
-  hipLaunchKernelGGL(k_to_grid, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), mode, Fs.p, tw.p, mu.p, out, b0);
+  LMW_DISPATCH_L(K / 2, k_to_grid, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), mode, Fs.p, tw.p, mu.p, out, b0);
   LMW_LAUNCH_CHECK();
 }
 
 void SphereCore::set_precision(int p) {
-  if (p != LMW_FP64) invalid("precision not available");
+  if (p != LMW_FP64 && p != LMW_FP32 && p != LMW_BF16X3) invalid("precision must be LMW_FP64, LMW_FP32 or LMW_BF16X3");
+  if (p != LMW_FP64) build_mixed_tables();
   precision = p;
+  if (graph) {
+    LMW_HIP(hipGraphExecDestroy(graph));
+    graph = nullptr;
+  }
 }
 
 void SphereCore::set_state_grid(int b, const double *h, const double *u, const double *v) {
@@ -785,12 +807,20 @@ void SphereCore::mark_reference(int b) {
                          hipMemcpyDeviceToDevice, stream));
 }
 
-void SphereCore::enqueue_step(double dt) {
-  for (int stage = 1; stage <= 4; stage++) {
+void SphereCore::launch_stage(int stage, double dt) {
+  if (precision == LMW_FP64) {
     launch_synth(xs.p, 0, B);
-    launch_tend_grid(0, B);
+    launch_tend_grid(0, B, 0);
     launch_anal(ANAL_TEND, stage, dt, 0, B, nullptr);
+  } else {  // only the Legendre contractions of the time step change precision
+    launch_synth_mx(xs.p, 0, B);
+    launch_tend_grid(0, B, 1);
+    launch_anal_mx(ANAL_TEND, stage, dt, 0, B, nullptr);
   }
+}
+
+void SphereCore::enqueue_step(double dt) {
+  for (int stage = 1; stage <= 4; stage++) launch_stage(stage, dt);
 }
 
 void SphereCore::steps(double dt, int64_t count) {
@@ -822,12 +852,15 @@ void SphereCore::steps_timed(double dt, int64_t count, double ms[3]) {
   size_t q = 0;
   for (int64_t i = 0; i < count; i++)
     for (int stage = 1; stage <= 4; stage++, q += 4) {
+      const bool mx = precision != LMW_FP64;
       LMW_HIP(hipEventRecord(ev[q], stream));
-      launch_synth(xs.p, 0, B);
+      if (mx) launch_synth_mx(xs.p, 0, B);
+      else launch_synth(xs.p, 0, B);
       LMW_HIP(hipEventRecord(ev[q + 1], stream));
-      launch_tend_grid(0, B);
+      launch_tend_grid(0, B, mx ? 1 : 0);
       LMW_HIP(hipEventRecord(ev[q + 2], stream));
-      launch_anal(ANAL_TEND, stage, dt, 0, B, nullptr);
+      if (mx) launch_anal_mx(ANAL_TEND, stage, dt, 0, B, nullptr);
+      else launch_anal(ANAL_TEND, stage, dt, 0, B, nullptr);
       LMW_HIP(hipEventRecord(ev[q + 3], stream));
     }
   sync();
@@ -899,9 +932,15 @@ void SphereCore::analyze_raw(int mode, const double *g0, const double *g1, const
 }
 
 void SphereCore::tendencies_raw(int b, double2 *dz, double2 *dd, double2 *df) {
-  launch_synth(x0.p, b, 1);
-  launch_tend_grid(b, 1);
-  launch_anal(ANAL_TENDOUT, 0, 0.0, b, 1, spec_io.p);
+  if (precision == LMW_FP64) {
+    launch_synth(x0.p, b, 1);
+    launch_tend_grid(b, 1, 0);
+    launch_anal(ANAL_TENDOUT, 0, 0.0, b, 1, spec_io.p);
+  } else {
+    launch_synth_mx(x0.p, b, 1);
+    launch_tend_grid(b, 1, 1);
+    launch_anal_mx(ANAL_TENDOUT, 0, 0.0, b, 1, spec_io.p);
+  }
   spec_io.download(dz, S, 0, stream);
   spec_io.download(dd, S, S, stream);
   spec_io.download(df, S, 2 * S, stream);
