@@ -1,13 +1,16 @@
-"""Benchmark: ensemble member-days/s at T170 on one MI355X.
+"""Benchmark: ensemble member-days/s at T170 on one MI355X, mixed precision vs FP64.
 
 Workload (one "step" = one RK4 step of the whole batch): ``--members`` members at T170 (512 x 256 Gaussian grid,
 a = 6.371e6 m, dt = 150 s) -- the T170 share of the 256-member uneven ensemble -- each the T170 random balanced state
 (energy up to n = 85, seed 1) over 8 seeded Gaussian mountains, with its vorticity coefficients perturbed by 1e-3
 relative noise (seed 1000 + member).  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line: value = whole-job member-days/s, the roofline of the dominant kernel measured with HIP events
-around every launch on the library's stream, the FP64 single-member ("plain") rate, and the reference CPU code timed
-on this host (oracle/_ref, one core) on a bounded sample of the same workload.
+value = whole-job member-days/s of the batch with the Legendre contractions in ``--precision`` (default split BF16
+hi+lo on MFMA); the same batch in FP64 and one FP64 member ("plain") are timed beside it.  The accuracy of the mixed
+precision is the rel-L2 of h, u, v against FP64 after ``--check-days`` simulated days of one member, next to an FP64
+twin started from a 1e-7 relative vorticity perturbation (the dynamics' own error growth).  The roofline is that of
+the dominant kernel, measured with HIP events around every launch on the library's stream.  The reference CPU code
+(oracle/_ref, one core) is timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -28,33 +31,47 @@ PEAKS = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 HBM_PEAK = 8.0e12
 
 
-def build_state(lmw, members, precision):
-    from lmweather_amd.initial import gaussian_orography, perturb_vorticity, random_balanced_state
-    p = lmw.SphereParams(A, OM, G)
-    base = lmw.Sphere(N, p)
+def initial_state(lmw):
+    from lmweather_amd.initial import gaussian_orography, random_balanced_state
+    base = lmw.Sphere(N, lmw.SphereParams(A, OM, G))
     h, u, v = random_balanced_state(base, 85, 1)
     hs = gaussian_orography(base, 8, 2)
-    ens = lmw.Ensemble([N] * members, p, precision)
+    return h - hs, u, v, hs
+
+
+def build_batch(lmw, members, precision, state):
+    from lmweather_amd.initial import perturb_vorticity
+    h, u, v, hs = state
+    ens = lmw.Ensemble([N] * members, lmw.SphereParams(A, OM, G), precision)
     for i in range(members):
         ens.set_orography_grid(i, hs)
-        ens.set_state_grid(i, h - hs, u, v)
+        ens.set_state_grid(i, h, u, v)
         ens.set_spectral(i, 0, perturb_vorticity(ens.get_spectral(i, 0), 1e-3, 1000 + i))
-    return ens, (h - hs, u, v, hs)
+    return ens
 
 
-def kernel_costs(J, K, S, M1, members):
-    """Algorithmic work per launch (one RK4 stage of the batch).  Legendre flops counted as executed with the
-    equatorial folding (half of the reference's 24 J S and 28 J S); the grid kernel's HBM bytes: Fourier
-    coefficients in (4 fields), f and hs grids in, analysis operands out."""
+def kernel_costs(J, K, S, M1, members, mixed):
+    """Algorithmic work per launch (one RK4 stage of the batch).  Legendre flops as executed with the equatorial
+    folding (half of the reference's 24 J S and 28 J S per member); for split BF16 the MFMA executes three products
+    per multiply-add.  Grid kernel HBM bytes: Fourier coefficients in, orography in, analysis operands out (FP32
+    storage in the mixed modes, FP64 otherwise)."""
     Jh = J // 2
+    cb = 8 if mixed else 16  # bytes per stored complex
     synth_flops = 24.0 * Jh * S * members
     anal_flops = 28.0 * Jh * S * members
-    grid_bytes = (4 * J * M1 * 16 + 2 * J * K * 8 + M1 * Jh * 16 * 16) * members
+    grid_bytes = (4 * J * M1 * cb + J * K * 8 + M1 * Jh * 16 * cb) * members
     return synth_flops, grid_bytes, anal_flops
 
 
+def timed_batch(ens, steps, warmup):
+    ens.steps([DT], [warmup])
+    t0 = time.perf_counter()
+    ms = ens.steps_timed([DT], [steps])[0]  # synchronous: returns once the device has finished
+    return time.perf_counter() - t0, ms
+
+
 def cpu_baseline(state, steps):
-    """The reference's own code (oracle/_ref, else the oracle restatement) on one host core, `steps` RK4 steps of
+    """The reference's own code (oracle/_ref, else the oracle restatement) on one host core: `steps` RK4 steps of
     one T170 member from the same initial state."""
     from oracle import pyapi
     kind = "reference" if pyapi.available("ref") else "port"
@@ -72,16 +89,93 @@ def cpu_baseline(state, steps):
             "ms_per_step": 1e3 * wall / steps}
 
 
+def accuracy(lmw, state, prec, days):
+    """rel-L2 of h, u, v of one member in `prec` and of an FP64 twin (1e-7 relative vorticity perturbation) against
+    FP64 after `days`, plus the energy and mass drift of each."""
+    from lmweather_amd.initial import perturb_vorticity
+    h, u, v, hs = state
+    n = int(round(days * 86400 / DT))
+    out = {}
+    runs = {}
+    for name, pr, twin in (("fp64", lmw.FP64, False), ("mixed", prec, False), ("twin", lmw.FP64, True)):
+        s = lmw.Sphere(N, lmw.SphereParams(A, OM, G))
+        s.set_orography_grid(hs)
+        s.set_state_grid(h, u, v)
+        if twin:
+            s.set_spectral(0, perturb_vorticity(s.get_spectral(0), 1e-7, 77))
+        if pr != lmw.FP64:
+            s.set_precision(pr)
+        e0, m0 = s.energy(), s.mean_height()
+        s.steps(DT, n)
+        runs[name] = (s.grid_height(), s.grid_u(), s.grid_v())
+        out[f"{name}_energy_drift"] = (s.energy() - e0) / e0
+        out[f"{name}_mass_drift"] = (s.mean_height() - m0) / m0
+    for name in ("mixed", "twin"):
+        for i, f in enumerate("huv"):
+            a, b = runs[name][i], runs["fp64"][i]
+            out[f"{name}_rel_l2_{f}"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["days"] = days
+    return out
+
+
+def plane_jet_state(nx, ny, lx, ly, g, H, f0, beta, U=1.0, Lw=1.0e5):
+    """Geostrophically balanced sech^2 jet: u = U sech^2(s) at u points, h = H - (f0 U L / g) tanh s
+    - (beta U L^2 / g)(s tanh s - ln cosh s) at centres, s = (y - ly/2) / L (config C4)."""
+    dy = ly / ny
+    y = (np.arange(ny) + 0.5) * dy
+    s = (y - 0.5 * ly) / Lw
+    u = np.repeat((U / np.cosh(s) ** 2)[:, None], nx, axis=1)
+    hrow = H - (f0 * U * Lw / g) * np.tanh(s) - (beta * U * Lw * Lw / g) * (s * np.tanh(s) - np.log(np.cosh(s)))
+    h = np.repeat(hrow[:, None], nx, axis=1)
+    return h, u
+
+
+def plane_bench(lmw, args) -> dict:
+    """Config C4: 1024 x 512 beta-plane channel (dx = dy = 10 km, f0 = 1e-4, beta = 1.6e-11, H = 1000 m), a batch of
+    members each the sech^2 jet plus white height noise of 1e-3 m (seed 3 + member), RK4 at dt = stable_dt(0.5)."""
+    from lmweather_amd.initial import normals
+    nx, ny, lx, ly, g, H, f0, beta = 1024, 512, 1.024e7, 5.12e6, 9.81, 1000.0, 1e-4, 1.6e-11
+    prec = lmw.FP64 if args.precision == "fp64" else lmw.FP32
+    p = lmw.PlaneParams(nx, ny, lx, ly, g, H, f0, beta)
+    pl = lmw.Plane(p, args.members, prec)
+    h, u = plane_jet_state(nx, ny, lx, ly, g, H, f0, beta)
+    for b in range(args.members):
+        pl.set_field(0, h + 1e-3 * normals(3 + b, nx * ny).reshape(ny, nx), b)
+        pl.set_field(1, u, b)
+    dt = pl.stable_dt(0.5)
+    m0 = pl.mean_depth(0)
+    pl.steps(dt, args.warmup)
+    t0 = time.perf_counter()
+    pl.steps(dt, args.steps)
+    wall = time.perf_counter() - t0
+    cells = nx * ny
+    bpc = 240.0 if prec == lmw.FP64 else 120.0  # canonical bytes per cell per RK4 step
+    gbs = bpc * cells * args.members * args.steps / wall / 1e9
+    return {
+        "metric": "beta-plane member-days/s (C-grid channel)", "value": args.members * dt * args.steps / 86400.0 / wall,
+        "unit": "member-days/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == lmw.FP64 else "f32", "data": "synthetic",
+        "config": {"workload": f"beta-plane channel 1024x512 (dx 10 km), sech^2 jet + 1e-3 m noise, batch {args.members}, "
+                               f"RK4 dt {dt:.2f} s", "members": args.members, "dt_s": dt},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": gbs * 1e9 / HBM_PEAK,
+                     "traffic": None, "kernel": "k_plane_stage", "work_per_launch": f"{bpc / 4 * cells * args.members / 1e6:.1f} MB"},
+        "mass_drift": (pl.mean_depth(0) - m0) / m0,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--members", type=int, default=32)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--check-days", type=float, default=1.0)
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the single-member FP64 rate (clean kernel profiles)")
+    ap.add_argument("--only-batch", action="store_true", help="time the batch only (clean kernel profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design, not a run",
@@ -89,53 +183,48 @@ def main() -> int:
         return 2
 
     import lmweather_amd as lmw
+    if args.workload == "plane":
+        print(json.dumps(plane_bench(lmw, args)))
+        return 0
     prec = {"fp64": lmw.FP64, "fp32": lmw.FP32, "bf16x3": lmw.BF16X3}[args.precision]
+    mixed = prec != lmw.FP64
     name, cus, _ = lmw.device_info()
     print(f"device {name} ({cus} CUs); building {args.members} T170 members", file=sys.stderr, flush=True)
-    ens, state = build_state(lmw, args.members, prec)
+    state = initial_state(lmw)
+    ens = build_batch(lmw, args.members, prec, state)
     _, J, K = ens.dims(0)
     S = (N + 1) * (N + 2) // 2
-    ens.steps([DT], [args.warmup])
-    print(f"warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
-
-    t0 = time.perf_counter()
-    ms = ens.steps_timed([DT], [args.steps])[0]  # synchronous: returns after the device finished
-    wall = time.perf_counter() - t0
+    wall, ms = timed_batch(ens, args.steps, args.warmup)
+    del ens
     value = args.members * DT * args.steps / 86400.0 / wall
-    print(f"timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+    print(f"{args.precision}: timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
 
     launches = 4 * args.steps
-    synth_f, grid_b, anal_f = kernel_costs(J, K, S, N + 1, args.members)
+    synth_f, grid_b, anal_f = kernel_costs(J, K, S, N + 1, args.members, mixed)
     avg = ms / launches * 1e-3  # seconds per launch
     peak = PEAKS[args.precision]
+    mult = 3.0 if args.precision == "bf16x3" else 1.0  # MFMA products per multiply-add
     kernels = {
-        "legendre_inverse": {"avg_us": avg[0] * 1e6, "tflops": synth_f / avg[0] / 1e12, "frac": synth_f / avg[0] / peak},
-        "grid_fft_products": {"avg_us": avg[1] * 1e6, "gbs": grid_b / avg[1] / 1e9, "frac": grid_b / avg[1] / HBM_PEAK},
-        "legendre_forward_rk4": {"avg_us": avg[2] * 1e6, "tflops": anal_f / avg[2] / 1e12, "frac": anal_f / avg[2] / peak},
+        "legendre_inverse": {"kernel": "k_synth_mx" if mixed else "k_synth", "avg_us": avg[0] * 1e6,
+                             "tflops": synth_f / avg[0] / 1e12, "frac": mult * synth_f / avg[0] / peak},
+        "grid_fft_products": {"kernel": "k_tend_grid", "avg_us": avg[1] * 1e6, "gbs": grid_b / avg[1] / 1e9,
+                              "frac": grid_b / avg[1] / HBM_PEAK},
+        "legendre_forward_rk4": {"kernel": "k_anal_mx" if mixed else "k_anal", "avg_us": avg[2] * 1e6,
+                                 "tflops": anal_f / avg[2] / 1e12, "frac": mult * anal_f / avg[2] / peak},
     }
     dom = int(np.argmax(ms))
     if dom == 1:
         roof = {"bound": "hbm", "achieved": grid_b / avg[1] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": grid_b / avg[1] / HBM_PEAK, "traffic": None, "kernel": "k_tend_grid"}
+                "frac": grid_b / avg[1] / HBM_PEAK, "traffic": None, "kernel": "k_tend_grid",
+                "work_per_launch": f"{grid_b / 1e6:.1f} MB"}
     else:
         fl = synth_f if dom == 0 else anal_f
         roof = {"bound": "mfma", "achieved": fl / avg[dom] / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": fl / avg[dom] / peak, "traffic": None, "kernel": "k_synth" if dom == 0 else "k_anal"}
-
-    plain = None
-    if not args.no_plain:  # FP64 single member ("plain" form of config C2)
-        p = lmw.SphereParams(A, OM, G)
-        one = lmw.Sphere(N, p)
-        h, u, v, hs = state
-        one.set_orography_grid(hs)
-        one.set_state_grid(h, u, v)
-        one.steps(DT, 4)
-        t1 = time.perf_counter()
-        one.steps(DT, args.steps)
-        plain = DT * args.steps / 86400.0 / (time.perf_counter() - t1)
+                "frac": mult * fl / avg[dom] / peak, "traffic": None, "kernel": kernels[list(kernels)[dom]]["kernel"],
+                "work_per_launch": f"{fl / 1e9:.3f} GFLOP"}
 
     out = {
-        "metric": "ensemble member-days/s at T170",
+        "metric": "ensemble member-days/s at T170, mixed precision vs FP64",
         "value": value,
         "unit": "member-days/s",
         "n_gpus": 1,
@@ -145,20 +234,40 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 Legendre / f64", "bf16x3": "bf16x3 Legendre / f64"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32 Legendre, f64 state/FFT",
+                  "bf16x3": "bf16x3 Legendre (f32 acc), f64 state/FFT"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"T170 ensemble, {args.members} members (512x256 Gaussian grid, dt {DT:g} s, RK4), "
                                "balanced random flow n<=85 over 8 Gaussian mountains, 1e-3 vorticity perturbations",
                    "members": args.members, "truncation": N, "dt_s": DT, "precision": args.precision},
         "roofline": roof,
         "kernels": kernels,
-        "fp64_plain_member_days_per_s": plain,
-        # the reference's Legendre flop count (52 J S per member and stage, twice the folded work executed here)
-        "reference_equivalent_legendre_tflops": 2 * (synth_f + anal_f) * launches / ((ms[0] + ms[2]) * 1e-3) / 1e12,
     }
-    if not args.no_cpu:
-        print(f"cpu baseline: {args.cpu_steps} reference steps", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(state, args.cpu_steps)
+    if not args.only_batch:
+        if mixed:
+            e64 = build_batch(lmw, args.members, lmw.FP64, state)
+            n64 = max(args.steps // 2, 4)
+            w64, _ = timed_batch(e64, n64, args.warmup)
+            del e64
+            out["fp64_batched_member_days_per_s"] = args.members * DT * n64 / 86400.0 / w64
+        p = lmw.SphereParams(A, OM, G)
+        one = lmw.Sphere(N, p)
+        h, u, v, hs = state
+        one.set_orography_grid(hs)
+        one.set_state_grid(h, u, v)
+        one.steps(DT, 4)
+        t1 = time.perf_counter()
+        one.steps(DT, args.steps)
+        out["fp64_plain_member_days_per_s"] = DT * args.steps / 86400.0 / (time.perf_counter() - t1)
+        out["speedup_vs_fp64_plain"] = value / out["fp64_plain_member_days_per_s"]
+        if "fp64_batched_member_days_per_s" in out:
+            out["speedup_vs_fp64_batched"] = value / out["fp64_batched_member_days_per_s"]
+        if mixed:
+            print(f"accuracy: {args.check_days} days", file=sys.stderr, flush=True)
+            out["accuracy_vs_fp64"] = accuracy(lmw, state, prec, args.check_days)
+        if not args.no_cpu:
+            print(f"cpu baseline: {args.cpu_steps} reference steps", file=sys.stderr, flush=True)
+            out["cpu_baseline"] = cpu_baseline(state, args.cpu_steps)
     print(json.dumps(out))
     return 0
 
