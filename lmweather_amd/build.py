@@ -62,6 +62,20 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+def build_cpp_tests() -> str:
+    """Compile tests/core_gpu.cpp (the reference's native checks restated on the C++ facade include/lmw.hpp) into
+    tests/bin/core_gpu, linked against liblmw_hip.so."""
+    out_dir = os.path.join(ROOT, "tests", "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "core_gpu")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "core_gpu.cpp"), "-L", HERE, "-llmw_hip", "-Wl,-rpath,$ORIGIN/../../lmweather_amd", "-o", exe]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"building {exe} failed:\n{r.stdout}")
+    return exe
+
+
 def _drain(procs) -> None:
     errors = []
     for src, p in procs:
@@ -77,3 +91,4 @@ def _drain(procs) -> None:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_tests())
