@@ -63,12 +63,20 @@ typedef int lmw_status;
 #define LMW_DIAG_ERROR_L2 6
 #define LMW_DIAG_ERROR_LINF 7
 #define LMW_DIAG_TIME 8
+/* tracer_mean/tracer_mass/tracer_min/tracer_max/tracer_error_l2/tracer_error_linf: cpp/sphere.hpp:58-64 */
+#define LMW_DIAG_TRACER_MEAN 9
+#define LMW_DIAG_TRACER_MASS 10
+#define LMW_DIAG_TRACER_MIN 11
+#define LMW_DIAG_TRACER_MAX 12
+#define LMW_DIAG_TRACER_ERROR_L2 13
+#define LMW_DIAG_TRACER_ERROR_LINF 14
 
 /* point evaluators of lmw_sphere_point (reference height_at/wind_at/vorticity_at: cpp/sphere.hpp:83-86) */
 #define LMW_POINT_HEIGHT 0
 #define LMW_POINT_U 1
 #define LMW_POINT_V 2
 #define LMW_POINT_VORTICITY 3
+#define LMW_POINT_TRACER 4 /* tracer_at: cpp/sphere.hpp:60 / sphere.cpp:287-299 */
 
 /* plane fields and diagnostics (reference cpp/plane.hpp:32-56) */
 #define LMW_PLANE_H 0
@@ -103,6 +111,7 @@ typedef struct {
 typedef struct lmw_sphere lmw_sphere;
 typedef struct lmw_plane lmw_plane;
 typedef struct lmw_ensemble lmw_ensemble;
+typedef struct lmw_corrector lmw_corrector;
 
 /* ------------------------------------------------------------------------------------------- library */
 const char *lmw_last_error(void);
@@ -153,6 +162,11 @@ lmw_status lmw_sphere_get_spectral(lmw_sphere *s, int field, double *re_im, size
 lmw_status lmw_sphere_set_spectral(lmw_sphere *s, int field, const double *re_im, size_t n_complex);
 /* mean_height/energy/kinetic_energy/potential_energy/potential_enstrophy/max_wind/height_error_l2/linf/time:
  * cpp/sphere.hpp:70-81,34 / sphere.cpp:510-590 */
+/* set_tracer_grid: cpp/sphere.hpp:53 / sphere.cpp:241-246 (single spheres; the tracer is stepped with the state by
+ * lmw_sphere_step/steps/run, sphere.cpp:407-436); set_advect_only: sphere.hpp:55; has_tracer/advect_only: :54,56 */
+lmw_status lmw_sphere_set_tracer_grid(lmw_sphere *s, const double *q, size_t n);
+lmw_status lmw_sphere_set_advect_only(lmw_sphere *s, int on);
+lmw_status lmw_sphere_tracer_flags(const lmw_sphere *s, int *has_tracer, int *advect_only);
 lmw_status lmw_sphere_diag(lmw_sphere *s, int which, double *out);
 /* spectrum(ke, pe): cpp/sphere.hpp:78 / sphere.cpp:536-551 (n = truncation + 1 entries each) */
 lmw_status lmw_sphere_spectrum(lmw_sphere *s, double *ke, double *pe, size_t n);
@@ -194,10 +208,64 @@ lmw_status lmw_ensemble_groups(const lmw_ensemble *e, int *ngroups, int *truncat
 lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t *counts);
 /* capture one RK4 step of every group into a HIP graph and replay it (hipGraph path); counts per group */
 lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts);
+/* `count` macro steps: in each, group g advances substeps[g] steps of dts[g] (e.g. 600 s = 1 x 600 s at T42,
+ * 2 x 300 s at T85, 4 x 150 s at T170).  One macro step is captured once as a HIP graph whose branches are the
+ * groups' streams, so the truncations run concurrently and the host launches one graph per macro step (no reference
+ * counterpart) */
+lmw_status lmw_ensemble_macro_steps(lmw_ensemble *e, const double *dts, const int *substeps, int64_t count);
+/* Tile shapes of the Legendre kernels, chosen per truncation group by timing every variant on the group's state
+ * (restored afterwards) and cached in $LMW_TUNE_CACHE (default ~/.cache/lmweather_amd/tiles_v1.txt) keyed by
+ * architecture, truncation, members and precision; force != 0 re-times.  tiles[4 g + 0..3] = variant indices of
+ * the FP64 inverse, FP64 forward, mixed inverse and mixed forward transforms of group g; best_ms[2 g + 0..1] = the
+ * chosen inverse and forward transform times (0 when taken from the cache).  No reference counterpart. */
+lmw_status lmw_ensemble_autotune(lmw_ensemble *e, int force, int *tiles, double *best_ms);
+lmw_status lmw_sphere_autotune(lmw_sphere *s, int force, int *tiles, double *best_ms);
+/* set the four tile variant indices of group g directly (tests, or a caller's own tuning) */
+lmw_status lmw_ensemble_set_tiles(lmw_ensemble *e, int group, const int *tiles);
+/* the shape of tile variant `index`: kind 0 = FP64 members per block (a); 1 = mixed inverse row chunk (a);
+ * 2 = mixed forward latitude chunk (a) and waves per block (b) */
+lmw_status lmw_tile_variants(int kind, int index, int *a, int *b);
 /* the same steps launched eagerly with a HIP event pair around every kernel, on the stream the kernels run on;
  * kernel_ms[3 g + 0..2] = total milliseconds of the inverse Legendre, grid-point (FFT + products) and forward
  * Legendre kernels of group g (measurement entry point, no reference counterpart) */
 lmw_status lmw_ensemble_steps_timed(lmw_ensemble *e, const double *dts, const int64_t *counts, double *kernel_ms);
+
+/* ------------------------------------------------------------------------------------------- learned corrector */
+/* No reference counterpart: a spectral-space network (4 dense layers of width 8..64 over the packed coefficients
+ * (m, n) of zeta, delta, phi, with per-degree mixing in the two inner layers; BF16 weights, FP32 accumulation)
+ * that corrects a coarse large-step integration toward a high-resolution model truncated to the same triangle.
+ * Spectra cross this boundary as float [sample][3][C][2] with C = (T+1)(T+2)/2 in the packed triangular order
+ * m-major, index m (T+1) - m (m-1)/2 + n - m.  The plain spectral model stays the fallback. */
+lmw_status lmw_corrector_create(int truncation, int width, uint64_t seed, lmw_corrector **out);
+lmw_status lmw_corrector_destroy(lmw_corrector *c);
+lmw_status lmw_corrector_dims(const lmw_corrector *c, int *truncation, int *width, size_t *nparams);
+/* the FP32 master parameters: W1 [W][8], b1 [W], W2 [W][W], b2 [W], V2 [W][W], W3 [W][W], b3 [W], V3 [W][W],
+ * W4 [6][W], b4 [6]; set_params also resets the Adam moments */
+lmw_status lmw_corrector_get_params(lmw_corrector *c, float *out, size_t n);
+lmw_status lmw_corrector_set_params(lmw_corrector *c, const float *in, size_t n);
+/* per-degree scales [3][T+1] (zeta, delta, phi): inputs are divided by in_scale, outputs multiplied by out_scale */
+lmw_status lmw_corrector_set_scales(lmw_corrector *c, const float *in_scale, const float *out_scale, size_t n);
+/* mean squared error (trained entries: all but the (0,0) row and the imaginary parts at m = 0) of the network on
+ * nsamples inputs against target corrections, and its gradient (grad may be NULL) */
+lmw_status lmw_corrector_loss_grad(lmw_corrector *c, const float *x, const float *target, int nsamples, double *loss, float *grad);
+/* the correction for nsamples inputs, and the rms of the network's outputs per sample (residual may be NULL) */
+lmw_status lmw_corrector_forward(lmw_corrector *c, const float *x, int nsamples, float *correction, double *residual);
+/* the training set, resident on the device: inputs and target corrections */
+lmw_status lmw_corrector_set_dataset(lmw_corrector *c, const float *x, const float *target, int nsamples);
+/* `epochs` passes of Adam (lr, beta 0.9/0.999) over the training set in shuffled mini-batches; losses[e] = mean
+ * loss of epoch e */
+lmw_status lmw_corrector_train(lmw_corrector *c, int epochs, int batch, float lr, uint64_t seed, double *losses);
+/* one corrected step of dt: coarse_substeps RK4 steps of dt / coarse_substeps, then the correction; a member whose
+ * residual exceeds `threshold` (or is not finite) is stepped from its start state with fallback_substeps plain RK4
+ * steps of dt / fallback_substeps instead.  used_fallback and residual (per member, may be NULL). */
+lmw_status lmw_sphere_corrected_step(lmw_sphere *s, lmw_corrector *c, double dt, int coarse_substeps, int fallback_substeps, double threshold,
+                                     int *used_fallback, double *residual);
+lmw_status lmw_ensemble_corrected_step(lmw_ensemble *e, lmw_corrector *c, double dt, int coarse_substeps, int fallback_substeps,
+                                       double threshold, int *used_fallback, double *residual);
+/* the spectral state of every member of a single-truncation ensemble truncated to T (a coefficient copy: the
+ * normalization does not depend on the truncation), float [member][3][C_T][2]; and setting it (zero above T) */
+lmw_status lmw_ensemble_get_truncated(lmw_ensemble *e, int truncation, float *out, size_t n);
+lmw_status lmw_ensemble_set_truncated(lmw_ensemble *e, int truncation, const float *in, size_t n);
 
 /* ------------------------------------------------------------------------------------------- plane */
 /* replaces Plane::Plane(const PlaneParams&), cpp/plane.hpp:22 / plane.cpp:20-25; `batch` independent members

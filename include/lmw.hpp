@@ -72,6 +72,27 @@ class Sphere {
   void perturb(double amplitude, uint64_t seed, int max_wave) { check(lmw_sphere_perturb(h_, amplitude, seed, max_wave)); }
   void set_orography_grid(const std::vector<double> &hs) { check(lmw_sphere_set_orography_grid(h_, hs.data(), hs.size())); }
   void set_precision(int precision) { check(lmw_sphere_set_precision(h_, precision)); }
+  // passive tracer (cpp/sphere.hpp:52-64)
+  void set_tracer_grid(const std::vector<double> &q) { check(lmw_sphere_set_tracer_grid(h_, q.data(), q.size())); }
+  void set_advect_only(bool on) { check(lmw_sphere_set_advect_only(h_, on ? 1 : 0)); }
+  bool has_tracer() const {
+    int t = 0;
+    check(lmw_sphere_tracer_flags(h_, &t, nullptr));
+    return t != 0;
+  }
+  bool advect_only() const {
+    int a = 0;
+    check(lmw_sphere_tracer_flags(h_, nullptr, &a));
+    return a != 0;
+  }
+  std::vector<double> grid_tracer() const { return grid(LMW_GRID_TRACER); }
+  double tracer_mean() const { return diag(LMW_DIAG_TRACER_MEAN); }
+  double tracer_mass() const { return diag(LMW_DIAG_TRACER_MASS); }
+  double tracer_at(double lat, double lon) const { return point(LMW_POINT_TRACER, lat, lon); }
+  double tracer_min() const { return diag(LMW_DIAG_TRACER_MIN); }
+  double tracer_max() const { return diag(LMW_DIAG_TRACER_MAX); }
+  double tracer_error_l2() const { return diag(LMW_DIAG_TRACER_ERROR_L2); }
+  double tracer_error_linf() const { return diag(LMW_DIAG_TRACER_ERROR_LINF); }
 
   void step(double dt) { check(lmw_sphere_step(h_, dt)); }
   void steps(double dt, int64_t count) { check(lmw_sphere_steps(h_, dt, count)); }

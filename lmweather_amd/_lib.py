@@ -17,8 +17,9 @@ FP64, FP32, BF16X3 = 0, 1, 2
 GRID_H, GRID_U, GRID_V, GRID_VORTICITY, GRID_TRACER = 0, 1, 2, 3, 4
 SPEC_ZETA, SPEC_DELTA, SPEC_PHI, SPEC_Q = 0, 1, 2, 3
 (DIAG_MEAN_HEIGHT, DIAG_ENERGY, DIAG_KINETIC, DIAG_POTENTIAL, DIAG_ENSTROPHY, DIAG_MAX_WIND, DIAG_ERROR_L2, DIAG_ERROR_LINF,
- DIAG_TIME) = range(9)
-POINT_HEIGHT, POINT_U, POINT_V, POINT_VORTICITY = 0, 1, 2, 3
+ DIAG_TIME, DIAG_TRACER_MEAN, DIAG_TRACER_MASS, DIAG_TRACER_MIN, DIAG_TRACER_MAX, DIAG_TRACER_ERROR_L2,
+ DIAG_TRACER_ERROR_LINF) = range(15)
+POINT_HEIGHT, POINT_U, POINT_V, POINT_VORTICITY, POINT_TRACER = 0, 1, 2, 3, 4
 PLANE_H, PLANE_U, PLANE_V = 0, 1, 2
 PDIAG_MEAN_DEPTH, PDIAG_ENERGY, PDIAG_MAX_SPEED, PDIAG_PV_ANOMALY_RMS, PDIAG_TIME = range(5)
 
@@ -52,6 +53,7 @@ _dp = C.POINTER(C.c_double)
 _vp = C.c_void_p
 _i64p = C.POINTER(C.c_int64)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 # name -> argtypes (every function returns lmw_status)
 SIGNATURES = {
@@ -76,6 +78,9 @@ SIGNATURES = {
     "lmw_sphere_get_grid": [_vp, C.c_int, _dp, C.c_size_t],
     "lmw_sphere_get_spectral": [_vp, C.c_int, _dp, C.c_size_t],
     "lmw_sphere_set_spectral": [_vp, C.c_int, _dp, C.c_size_t],
+    "lmw_sphere_set_tracer_grid": [_vp, _dp, C.c_size_t],
+    "lmw_sphere_set_advect_only": [_vp, C.c_int],
+    "lmw_sphere_tracer_flags": [_vp, _ip, _ip],
     "lmw_sphere_diag": [_vp, C.c_int, _dp],
     "lmw_sphere_spectrum": [_vp, _dp, _dp, C.c_size_t],
     "lmw_sphere_point": [_vp, C.c_int, C.c_double, C.c_double, _dp],
@@ -102,6 +107,25 @@ SIGNATURES = {
     "lmw_ensemble_steps": [_vp, _dp, _i64p],
     "lmw_ensemble_steps_graph": [_vp, _dp, _i64p],
     "lmw_ensemble_steps_timed": [_vp, _dp, _i64p, _dp],
+    "lmw_ensemble_macro_steps": [_vp, _dp, _ip, C.c_int64],
+    "lmw_ensemble_autotune": [_vp, C.c_int, _ip, _dp],
+    "lmw_sphere_autotune": [_vp, C.c_int, _ip, _dp],
+    "lmw_tile_variants": [C.c_int, C.c_int, _ip, _ip],
+    "lmw_ensemble_set_tiles": [_vp, C.c_int, _ip],
+    "lmw_corrector_create": [C.c_int, C.c_int, C.c_uint64, C.POINTER(_vp)],
+    "lmw_corrector_destroy": [_vp],
+    "lmw_corrector_dims": [_vp, _ip, _ip, C.POINTER(C.c_size_t)],
+    "lmw_corrector_get_params": [_vp, _fp, C.c_size_t],
+    "lmw_corrector_set_params": [_vp, _fp, C.c_size_t],
+    "lmw_corrector_set_scales": [_vp, _fp, _fp, C.c_size_t],
+    "lmw_corrector_loss_grad": [_vp, _fp, _fp, C.c_int, _dp, _fp],
+    "lmw_corrector_forward": [_vp, _fp, C.c_int, _fp, _dp],
+    "lmw_corrector_set_dataset": [_vp, _fp, _fp, C.c_int],
+    "lmw_corrector_train": [_vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _dp],
+    "lmw_sphere_corrected_step": [_vp, _vp, C.c_double, C.c_int, C.c_int, C.c_double, _ip, _dp],
+    "lmw_ensemble_corrected_step": [_vp, _vp, C.c_double, C.c_int, C.c_int, C.c_double, _ip, _dp],
+    "lmw_ensemble_get_truncated": [_vp, C.c_int, _fp, C.c_size_t],
+    "lmw_ensemble_set_truncated": [_vp, C.c_int, _fp, C.c_size_t],
     "lmw_plane_create": [C.POINTER(PlaneParams), C.c_int, C.c_int, C.POINTER(_vp)],
     "lmw_plane_destroy": [_vp],
     "lmw_plane_set_params": [_vp, C.POINTER(PlaneParams)],
@@ -158,6 +182,14 @@ def call(name: str, *args) -> None:
 
 def dptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(_fp)
+
+
+def as_f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 def as_f64(a) -> np.ndarray:

@@ -106,10 +106,12 @@ struct SynthTabs {
 // Block: one m, 16 members, one wave per 16 northern latitudes.  Coefficients are staged in LDS in chunks of 32
 // rows of one parity: quantities q = zeta re/im, phi re/im, psi re/im, chi re/im (H uses psi, chi = q 4..7).
 // LDS per chunk: FP32 [q][k'][member] floats; BF16 [q][member][k'] hi and lo.
-template <int PREC>
+// Tile shape (autotuned per truncation group): RC rows of one parity per LDS chunk (a multiple of 32, the K of one
+// BF16 MFMA).
+template <int PREC, int RC>
 __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
                                                   int nb, int mixed) {
-  constexpr int SP = 40;  // padded row pitch (bf16) of the BF16 chunk: distinct LDS banks across members
+  constexpr int SP = RC + 8;  // padded row pitch (bf16) of the BF16 chunk: distinct LDS banks across members
   extern __shared__ float lds_f[];
   float *cf = lds_f;                                                       // FP32 chunk [q][k'][member]
   __bf16 *ch = reinterpret_cast<__bf16 *>(lds_f), *cl = ch + 8 * MEM * SP;  // BF16 chunk [q][member][k'], hi and lo
@@ -127,11 +129,11 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
   for (int p = 0; p < 2; p++) {
     const int kp = p ? kpo : kpe;             // rows of this parity (16-padded)
     const int kp32 = PREC == LMW_BF16X3 ? (p ? T.ko32[m] : T.ke32[m]) : kp;
-    for (int c0 = 0; c0 < kp32; c0 += 32) {
+    for (int c0 = 0; c0 < kp32; c0 += RC) {
       __syncthreads();
-      // stage 8 quantities x 32 rows x 16 members: one thread per (member, pair of rows), members fastest, so
+      // stage 8 quantities x RC rows x 16 members: one thread per (member, pair of rows), members fastest, so
       // that a wavefront writes consecutive LDS words (FP32) or whole words of a padded pitch (BF16)
-      for (int e = threadIdx.x; e < 16 * MEM; e += blockDim.x) {
+      for (int e = threadIdx.x; e < (RC / 2) * MEM; e += blockDim.x) {
         const int j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
         float v[2][8];
 #pragma unroll
@@ -158,8 +160,8 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
 #pragma unroll
         for (int q = 0; q < 8; q++) {
           if (PREC == LMW_FP32) {
-            cf[(q * 32 + kk) * MEM + j] = v[0][q];
-            cf[(q * 32 + kk + 1) * MEM + j] = v[1][q];
+            cf[(q * RC + kk) * MEM + j] = v[0][q];
+            cf[(q * RC + kk + 1) * MEM + j] = v[1][q];
           } else {
             __bf16 h0, l0, h1, l1;
             split_bf16(v[0][q], h0, l0);
@@ -175,28 +177,31 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
       const int jh = jt * 16 + (lane & 15);
       if (PREC == LMW_FP32) {
         const int rbase = G.poff[m] + (p ? kpe : 0);
-        const int steps = min(32, kp - c0) / 4;
+        const int steps = max(0, min(RC, kp - c0)) / 4;
         for (int s = 0; s < steps; s++) {
           const int kk = 4 * s + (lane >> 4);
           const size_t ai = static_cast<size_t>(rbase + c0 + kk) * G.Jhp + jh;
           const float aP = T.Ps32[ai], aH = T.Hs32[ai];
 #pragma unroll
           for (int q = 0; q < 8; q++) {
-            const float bq = cf[(q * 32 + kk) * MEM + (lane & 15)];
+            const float bq = cf[(q * RC + kk) * MEM + (lane & 15)];
             accP[p][q] = mma32(aP, bq, accP[p][q]);
             if (q >= 4) accH[p][q - 4] = mma32(aH, bq, accH[p][q - 4]);
           }
         }
       } else {
-        const size_t base = static_cast<size_t>(T.soff[m]) + (p ? static_cast<size_t>(G.Jhp) * T.ke32[m] : 0) +
-                            static_cast<size_t>(jh) * kp32 + c0 + 8 * (lane >> 4);
-        const v8bf aPh = load8(T.Sph + base), aPl = load8(T.Spl + base), aHh = load8(T.Shh + base), aHl = load8(T.Shl + base);
+        const int nk = min(RC, kp32 - c0) / 32;  // MFMA K-steps of this chunk (kp32 is a multiple of 32)
+        for (int s = 0; s < nk; s++) {
+          const size_t base = static_cast<size_t>(T.soff[m]) + (p ? static_cast<size_t>(G.Jhp) * T.ke32[m] : 0) +
+                              static_cast<size_t>(jh) * kp32 + c0 + 32 * s + 8 * (lane >> 4);
+          const v8bf aPh = load8(T.Sph + base), aPl = load8(T.Spl + base), aHh = load8(T.Shh + base), aHl = load8(T.Shl + base);
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const int bo = (q * MEM + (lane & 15)) * SP + 8 * (lane >> 4);
-          const v8bf bh = load8(ch + bo), bl = load8(cl + bo);
-          accP[p][q] = mma3(aPh, aPl, bh, bl, accP[p][q]);
-          if (q >= 4) accH[p][q - 4] = mma3(aHh, aHl, bh, bl, accH[p][q - 4]);
+          for (int q = 0; q < 8; q++) {
+            const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
+            const v8bf bh = load8(ch + bo), bl = load8(cl + bo);
+            accP[p][q] = mma3(aPh, aPl, bh, bl, accP[p][q]);
+            if (q >= 4) accH[p][q - 4] = mma3(aHh, aHl, bh, bl, accH[p][q - 4]);
+          }
         }
       }
     }
@@ -239,11 +244,14 @@ struct AnalTabs {
 // Block: one (m, parity), 16 members, 4 waves over the 16-row tiles of that parity.  The K dimension runs over the
 // northern latitudes twice (P with X, then H with Y), staged in LDS in chunks of 64.  Quantities q = o1 re/im ..
 // o4 re/im (dzeta, ddelta, the Laplacian part of ddelta, dphi).
-template <int PREC>
-__global__ __launch_bounds__(256) void k_anal_mx(Geo G, AnalTabs T, const float2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
+// Tile shape (autotuned per truncation group): KC latitudes of K per LDS chunk, NW waves per block, each wave
+// owning row tiles wave, wave + NW, ... (TPW of them: 8 tiles of 16 rows cover the largest parity block, T170).
+template <int PREC, int KC, int NW>
+__global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
                                                  double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
                                                  double2 *__restrict__ raw, int b0, int nb, int mixed) {
-  constexpr int KC = 64, AP = 72;  // chunk of K; padded row pitch (bf16) of the BF16 chunk
+  constexpr int AP = KC + 8;         // padded row pitch (bf16) of the BF16 chunk
+  constexpr int TPW = (8 + NW - 1) / NW;
   extern __shared__ float lds_f[];
   float *bf = lds_f;                                                         // FP32 chunk [q][k][member]
   __bf16 *bh = reinterpret_cast<__bf16 *>(lds_f), *blo = bh + 8 * MEM * AP;  // BF16 chunk [q][member][k], hi and lo
@@ -252,8 +260,8 @@ __global__ __launch_bounds__(256) void k_anal_mx(Geo G, AnalTabs T, const float2
   const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Rp = p ? G.kpo[m] : kpe, rbase = p ? kpe : 0;
   if (Rp == 0) return;
   const int nrt = Rp / 16, KK = 2 * G.Jhp;
-  v4f accv[2][8];
-  for (int t = 0; t < 2; t++)
+  v4f accv[TPW][8];
+  for (int t = 0; t < TPW; t++)
     for (int q = 0; q < 8; q++) accv[t][q] = v4f{0, 0, 0, 0};
   for (int c0 = 0; c0 < KK; c0 += KC) {
     __syncthreads();
@@ -292,8 +300,9 @@ __global__ __launch_bounds__(256) void k_anal_mx(Geo G, AnalTabs T, const float2
       }
     }
     __syncthreads();
-    for (int ti = 0; ti < 2; ti++) {
-      const int t = wave + 4 * ti;
+#pragma unroll
+    for (int ti = 0; ti < TPW; ti++) {
+      const int t = wave + NW * ti;
       if (t >= nrt) break;
       const int r = rbase + 16 * t + (lane & 15);
       const int kc = min(KC, KK - c0);  // KK is a multiple of 32
@@ -320,8 +329,9 @@ __global__ __launch_bounds__(256) void k_anal_mx(Geo G, AnalTabs T, const float2
   const int j = lane & 15, bl = mb + j;
   if (bl >= nb) return;
   const int b = b0 + bl, Km = G.N - m + 1;
-  for (int ti = 0; ti < 2; ti++) {
-    const int t = wave + 4 * ti;
+#pragma unroll
+  for (int ti = 0; ti < TPW; ti++) {
+    const int t = wave + NW * ti;
     if (t >= nrt) break;
     for (int r = 0; r < 4; r++) {
       const int rl = 16 * t + 4 * (lane >> 4) + r;  // row within the parity block
@@ -378,28 +388,57 @@ void SphereCore::build_mixed_tables() {
   mixed_ready = true;
 }
 
-void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
-  SynthTabs T{Ps32.p, Hs32.p, Sph.p, Spl.p, Shh.p, Shl.p, d_soff.p, d_ke32.p, d_ko32.p};
-  const int waves = (Jh + 15) / 16;
-  dim3 grd(M1, (nb + MEM - 1) / MEM), blk(64 * waves);
-  if (precision == LMW_FP32)
-    hipLaunchKernelGGL(k_synth_mx<LMW_FP32>, grd, blk, 8 * 32 * MEM * sizeof(float), stream, geo(), T, spec, Fs32.p, b0, nb, 1);
-  else
-    hipLaunchKernelGGL(k_synth_mx<LMW_BF16X3>, grd, blk, 2 * 8 * 40 * MEM * sizeof(__bf16), stream, geo(), T, spec, Fs32.p, b0, nb, 1);
+// tile variants: synthesis row chunk RC; analysis (KC, NW)
+const int kSynthRC[kSynthTiles] = {32, 64};
+const int kAnalKC[kAnalTiles] = {32, 64, 128, 32, 64, 128};
+const int kAnalNW[kAnalTiles] = {4, 4, 4, 8, 8, 8};
+
+template <int PREC, int RC>
+static void synth_mx_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
+  SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
+  const int waves = (c.Jh + 15) / 16;
+  dim3 grd(c.M1, (nb + MEM - 1) / MEM), blk(64 * waves);
+  const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : 2 * 8 * (RC + 8) * MEM * sizeof(__bf16);
+  hipLaunchKernelGGL((k_synth_mx<PREC, RC>), grd, blk, lds, c.stream, c.geo(), T, spec, c.Fs32.p, b0, nb, 1);
   LMW_LAUNCH_CHECK();
 }
 
-void SphereCore::launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
-  AnalTabs T{A32.p, Abh.p, Abl.p};
-  dim3 grd(2 * M1, (nb + MEM - 1) / MEM), blk(256);
-  const size_t lds = precision == LMW_FP32 ? 8 * 64 * MEM * sizeof(float) : 2 * 8 * 72 * MEM * sizeof(__bf16);
-  if (precision == LMW_FP32)
-    hipLaunchKernelGGL(k_anal_mx<LMW_FP32>, grd, blk, lds, stream, geo(), T, Gb32.p, mode, stage, dt, prm.diffusion, x0.p, xs.p, acc.p, raw_out, b0,
-                       nb, 1);
-  else
-    hipLaunchKernelGGL(k_anal_mx<LMW_BF16X3>, grd, blk, lds, stream, geo(), T, Gb32.p, mode, stage, dt, prm.diffusion, x0.p, xs.p, acc.p, raw_out,
-                       b0, nb, 1);
+void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
+  const int rc = kSynthRC[tile_synth];
+  if (precision == LMW_FP32) {
+    if (rc == 64) synth_mx_launch<LMW_FP32, 64>(*this, spec, b0, nb);
+    else synth_mx_launch<LMW_FP32, 32>(*this, spec, b0, nb);
+  } else {
+    if (rc == 64) synth_mx_launch<LMW_BF16X3, 64>(*this, spec, b0, nb);
+    else synth_mx_launch<LMW_BF16X3, 32>(*this, spec, b0, nb);
+  }
+}
+
+template <int PREC, int KC, int NW>
+static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p};
+  dim3 grd(2 * c.M1, (nb + MEM - 1) / MEM), blk(64 * NW);
+  const size_t lds = PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float) : 2 * 8 * (KC + 8) * MEM * sizeof(__bf16);
+  hipLaunchKernelGGL((k_anal_mx<PREC, KC, NW>), grd, blk, lds, c.stream, c.geo(), T, c.Gb32.p, mode, stage, dt, c.prm.diffusion, c.x0.p, c.xs.p,
+                     c.acc.p, raw_out, b0, nb, 1);
   LMW_LAUNCH_CHECK();
+}
+
+template <int PREC>
+static void anal_mx_dispatch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  switch (c.tile_anal) {
+    case 0: anal_mx_launch<PREC, 32, 4>(c, mode, stage, dt, b0, nb, raw_out); break;
+    case 2: anal_mx_launch<PREC, 128, 4>(c, mode, stage, dt, b0, nb, raw_out); break;
+    case 3: anal_mx_launch<PREC, 32, 8>(c, mode, stage, dt, b0, nb, raw_out); break;
+    case 4: anal_mx_launch<PREC, 64, 8>(c, mode, stage, dt, b0, nb, raw_out); break;
+    case 5: anal_mx_launch<PREC, 128, 8>(c, mode, stage, dt, b0, nb, raw_out); break;
+    default: anal_mx_launch<PREC, 64, 4>(c, mode, stage, dt, b0, nb, raw_out); break;
+  }
+}
+
+void SphereCore::launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  if (precision == LMW_FP32) anal_mx_dispatch<LMW_FP32>(*this, mode, stage, dt, b0, nb, raw_out);
+  else anal_mx_dispatch<LMW_BF16X3>(*this, mode, stage, dt, b0, nb, raw_out);
 }
 
 }  // namespace lmw

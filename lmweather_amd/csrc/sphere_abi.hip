@@ -67,6 +67,12 @@ struct MemberView {
   }
   std::vector<double2> spectral(int field) const {
     std::vector<double2> p(c->S);
+    if (field == LMW_SPEC_Q) {  // the tracer lives in its own buffer (single-member cores)
+      if (!c->has_tracer) runtime("no tracer: set one with tracer() first");
+      c->q0.download(p.data(), c->S, 0, c->stream);
+      c->sync();
+      return p;
+    }
     c->get_spectral_packed(b, field, p.data());
     return p;
   }
@@ -80,13 +86,17 @@ struct MemberView {
     c->download_red(b, r);
     const double wmax = r[3], hmax = r[4];
     if (!(hmax > 0)) runtime("the height field must be positive to step the model");
-    const double omega = (c->N / c->prm.radius) * (std::sqrt(c->prm.gravity * hmax) + wmax) + 2 * c->prm.omega;
+    // advect-only: only the tracer moves, at the frozen wind speed (sphere.cpp:443-444)
+    const double omega = c->advect_only ? (c->N / c->prm.radius) * wmax
+                                        : (c->N / c->prm.radius) * (std::sqrt(c->prm.gravity * hmax) + wmax) + 2 * c->prm.omega;
     if (!(omega > 0)) return 1e9;
     return safety / omega;
   }
   // check_finite (sphere.cpp:449-455)
   void check_finite() {
-    if (!std::isfinite(c->spectral_abs_sum(b))) runtime("the integration blew up (not finite): use a smaller dt");
+    double sum = c->spectral_abs_sum(b);
+    if (c->has_tracer) sum += c->tracer_abs_sum();
+    if (!std::isfinite(sum)) runtime("the integration blew up (not finite): use a smaller dt");
   }
   void diag_sums(double out[16]) {
     c->compute_grids(b, 1);
@@ -108,11 +118,68 @@ struct MemberView {
       case LMW_DIAG_ERROR_L2: diag_sums(r); return std::sqrt(r[5] / r[6]);
       case LMW_DIAG_ERROR_LINF: diag_sums(r); return r[7] / r[8];
       case LMW_DIAG_TIME: return c->t[b];
+      case LMW_DIAG_TRACER_MEAN: {  // sphere.cpp:271-274
+        std::vector<double2> q = spectral(3);
+        return q[0].x / std::sqrt(2.0);
+      }
+      case LMW_DIAG_TRACER_MASS: return tracer_mass();
+      case LMW_DIAG_TRACER_MIN:
+      case LMW_DIAG_TRACER_MAX: {  // sphere.cpp:284-285
+        std::vector<double> g(ngrid());
+        c->tracer_grid(g.data());
+        return which == LMW_DIAG_TRACER_MIN ? *std::min_element(g.begin(), g.end()) : *std::max_element(g.begin(), g.end());
+      }
+      case LMW_DIAG_TRACER_ERROR_L2:
+      case LMW_DIAG_TRACER_ERROR_LINF: {
+        double l2, linf;
+        tracer_errors(l2, linf);
+        return which == LMW_DIAG_TRACER_ERROR_L2 ? l2 : linf;
+      }
       default: invalid("unknown diagnostic");
     }
   }
+  // tracer_mass (sphere.cpp:275-283): area mean of h q, row sums in the reference's order
+  double tracer_mass() {
+    if (!c->has_tracer) runtime("no tracer: set one with tracer() first");
+    std::vector<double> h(ngrid()), q(ngrid()), w(c->J);
+    grid(LMW_GRID_H, h.data());
+    c->tracer_grid(q.data());
+    c->w.download(w.data(), c->J, 0, c->stream);
+    c->sync();
+    double s = 0;
+    for (int j = 0; j < c->J; j++) {
+      double row = 0;
+      for (int k = 0; k < c->K; k++) row += h[static_cast<size_t>(j) * c->K + k] * q[static_cast<size_t>(j) * c->K + k];
+      s += w[j] * row / c->K;
+    }
+    return 0.5 * s;
+  }
+  // tracer_error_l2/linf (sphere.cpp:571-589 with g = 1 against the reference tracer grid)
+  void tracer_errors(double &l2, double &linf) {
+    if (!c->has_tracer) runtime("no tracer: set one with tracer() first");
+    std::vector<double> q(ngrid()), r(ngrid()), w(c->J);
+    c->tracer_grid(q.data());
+    c->ref_q.download(r.data(), ngrid(), 0, c->stream);
+    c->w.download(w.data(), c->J, 0, c->stream);
+    c->sync();
+    double num = 0, den = 0, mn = 0, mr = 0;
+    for (int j = 0; j < c->J; j++)
+      for (int k = 0; k < c->K; k++) {
+        const size_t i = static_cast<size_t>(j) * c->K + k;
+        const double rr = r[i] / 1.0, d = q[i] - rr;
+        num += w[j] * d * d;
+        den += w[j] * rr * rr;
+        mn = std::max(mn, std::fabs(d));
+        mr = std::max(mr, std::fabs(rr));
+      }
+    l2 = std::sqrt(num / den);
+    linf = mn / mr;
+  }
   void grid(int field, double *out) {
-    if (field == LMW_GRID_TRACER) runtime("no tracer: set one with tracer() first");
+    if (field == LMW_GRID_TRACER) {
+      c->tracer_grid(out);
+      return;
+    }
     if (field < 0 || field > 3) invalid("unknown grid field");
     c->compute_grids(b, 1);
     c->download_grid(b, field, out);
@@ -137,14 +204,15 @@ struct MemberView {
   // height_at / wind_at / vorticity_at (sphere.cpp:592-637)
   double point(int which, double lat, double lon) {
     const double mu = std::sin(lat), a = c->prm.radius;
-    std::vector<double2> z = spectral(0), d = spectral(1), f = spectral(2);
+    if (which == LMW_POINT_TRACER && !c->has_tracer) runtime("no tracer: set one with tracer() first");
+    std::vector<double2> z = spectral(0), d = spectral(1), f = which == LMW_POINT_TRACER ? spectral(3) : spectral(2);
     std::vector<double> col;
     double sum = 0;
     for (int m = 0; m <= N(); m++) {
       legendre_column_host(m, N(), mu, col);
       double gr, gi;
-      if (which == LMW_POINT_HEIGHT || which == LMW_POINT_VORTICITY) {
-        const std::vector<double2> &s = which == LMW_POINT_HEIGHT ? f : z;
+      if (which == LMW_POINT_HEIGHT || which == LMW_POINT_VORTICITY || which == LMW_POINT_TRACER) {
+        const std::vector<double2> &s = which == LMW_POINT_VORTICITY ? z : f;
         double sr = 0, si = 0;
         for (int n = m; n <= N(); n++) {
           const double2 v = s[c->packed_index(m, n)];
@@ -176,6 +244,7 @@ struct MemberView {
   // run (sphere.cpp:457-484)
   void run(double seconds, double dt) {
     if (!(seconds >= 0)) invalid("run: seconds must not be negative");
+    if (c->advect_only && !c->has_tracer) invalid("advect_only: set a tracer first (Sphere.tracer)");
     if (seconds == 0) return;
     if (!(dt > 0)) dt = stable_dt(1.5);
     const double limit = stable_dt(2.0 * std::sqrt(2.0));
@@ -264,6 +333,18 @@ struct lmw_sphere {
 struct lmw_ensemble {
   std::vector<std::unique_ptr<SphereCore>> groups;  // ascending truncation
   std::vector<std::pair<int, int>> where;           // member -> (group, local index)
+  // macro step: every group advanced by its own sub-steps, captured as one HIP graph whose branches are the groups'
+  // streams (forked from and joined to `origin`)
+  hipStream_t origin = nullptr;
+  hipGraphExec_t macro = nullptr;
+  std::vector<double> macro_dts;
+  std::vector<int> macro_sub;
+  std::vector<hipEvent_t> events;  // fork, then one join per group
+  ~lmw_ensemble() {
+    if (macro) (void)hipGraphExecDestroy(macro);
+    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
+    if (origin) (void)hipStreamDestroy(origin);
+  }
   MemberView view(int member) {
     if (member < 0 || member >= static_cast<int>(where.size())) invalid("member index out of range");
     return MemberView{groups[where[member].first].get(), where[member].second};
@@ -432,7 +513,8 @@ lmw_status lmw_sphere_get_spectral(lmw_sphere *s, int field, double *re_im, size
   return guard([&] {
     NEED(s);
     MemberView v = s->view();
-    if (field < 0 || field > 2) invalid(field == LMW_SPEC_Q ? "no tracer: set one with tracer() first" : "unknown spectral field");
+    if (field < 0 || field > 3) invalid("unknown spectral field");
+    if (field == LMW_SPEC_Q && !s->core->has_tracer) runtime("no tracer: set one with tracer() first");
     if (n_complex != v.nfull()) invalid("spectral size mismatch");
     NEED(re_im);
     v.from_packed(v.spectral(field), re_im);
@@ -442,13 +524,35 @@ lmw_status lmw_sphere_set_spectral(lmw_sphere *s, int field, const double *re_im
   return guard([&] {
     NEED(s);
     MemberView v = s->view();
-    if (field < 0 || field > 2) invalid("unknown spectral field");
+    if (field < 0 || field > 3) invalid("unknown spectral field");
     if (n_complex != v.nfull()) invalid("spectral size mismatch");
     NEED(re_im);
     std::vector<double2> p;
     v.to_packed(re_im, p);
-    v.set_spectral(field, p);
+    if (field == LMW_SPEC_Q) s->core->set_tracer_spectral(p.data());  // a tracer from its coefficients
+    else v.set_spectral(field, p);
     s->core->sync();
+  });
+}
+lmw_status lmw_sphere_set_tracer_grid(lmw_sphere *s, const double *q, size_t n) {
+  return guard([&] {
+    NEED(s);
+    if (n != s->view().ngrid()) invalid("grid size mismatch");
+    NEED(q);
+    s->core->set_tracer_grid(q);
+  });
+}
+lmw_status lmw_sphere_set_advect_only(lmw_sphere *s, int on) {
+  return guard([&] {
+    NEED(s);
+    s->core->set_advect_only(on != 0);
+  });
+}
+lmw_status lmw_sphere_tracer_flags(const lmw_sphere *s, int *has_tracer, int *advect_only) {
+  return guard([&] {
+    NEED(s);
+    if (has_tracer) *has_tracer = s->core->has_tracer ? 1 : 0;
+    if (advect_only) *advect_only = s->core->advect_only ? 1 : 0;
   });
 }
 lmw_status lmw_sphere_diag(lmw_sphere *s, int which, double *out) {
@@ -468,7 +572,7 @@ lmw_status lmw_sphere_spectrum(lmw_sphere *s, double *ke, double *pe, size_t n) 
 lmw_status lmw_sphere_point(lmw_sphere *s, int which, double lat, double lon, double *out) {
   return guard([&] {
     NEED(s && out);
-    if (which < 0 || which > 3) invalid("unknown point evaluator");
+    if (which < 0 || which > LMW_POINT_TRACER) invalid("unknown point evaluator");
     *out = s->view().point(which, lat, lon);
   });
 }
@@ -709,6 +813,59 @@ lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t 
 lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts) {
   return guard([&] { ensemble_steps(e, dts, counts, true); });
 }
+// one graph per macro step: group g advances substeps[g] steps of dts[g]; the groups' branches run concurrently
+static void capture_macro(lmw_ensemble *e, const double *dts, const int *substeps) {
+  const size_t ng = e->groups.size();
+  if (!e->origin) {
+    LMW_HIP(hipStreamCreate(&e->origin));
+    e->events.resize(ng + 1);
+    for (auto &ev : e->events) LMW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  if (e->macro) {
+    LMW_HIP(hipGraphExecDestroy(e->macro));
+    e->macro = nullptr;
+  }
+  hipGraph_t g;
+  LMW_HIP(hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
+  LMW_HIP(hipEventRecord(e->events[0], e->origin));
+  for (size_t i = 0; i < ng; i++) {
+    SphereCore &c = *e->groups[i];
+    LMW_HIP(hipStreamWaitEvent(c.stream, e->events[0], 0));
+    for (int k = 0; k < substeps[i]; k++) c.enqueue_step(dts[i]);
+    LMW_HIP(hipEventRecord(e->events[i + 1], c.stream));
+    LMW_HIP(hipStreamWaitEvent(e->origin, e->events[i + 1], 0));
+  }
+  LMW_HIP(hipStreamEndCapture(e->origin, &g));
+  LMW_HIP(hipGraphInstantiate(&e->macro, g, nullptr, nullptr, 0));
+  LMW_HIP(hipGraphDestroy(g));
+  e->macro_dts.assign(dts, dts + ng);
+  e->macro_sub.assign(substeps, substeps + ng);
+}
+lmw_status lmw_ensemble_macro_steps(lmw_ensemble *e, const double *dts, const int *substeps, int64_t count) {
+  return guard([&] {
+    NEED(e && dts && substeps);
+    const size_t ng = e->groups.size();
+    if (count < 0) invalid("step count must not be negative");
+    for (size_t i = 0; i < ng; i++) {
+      if (substeps[i] < 0) invalid("step count must not be negative");
+      if (substeps[i] > 0 && !(dts[i] > 0)) invalid("dt must be positive");
+      if (e->groups[i]->has_tracer) invalid("macro steps do not carry tracers");
+    }
+    if (count == 0) return;
+    for (auto &c : e->groups) c->sync();
+    bool same = e->macro && e->macro_sub.size() == ng;
+    for (size_t i = 0; same && i < ng; i++) same = e->macro_dts[i] == dts[i] && e->macro_sub[i] == substeps[i];
+    if (!same) capture_macro(e, dts, substeps);
+    for (int64_t k = 0; k < count; k++) LMW_HIP(hipGraphLaunch(e->macro, e->origin));
+    LMW_HIP(hipStreamSynchronize(e->origin));
+    for (size_t i = 0; i < ng; i++) {
+      SphereCore &c = *e->groups[i];
+      for (int b = 0; b < c.B; b++)
+        for (int64_t k = 0; k < count; k++)
+          for (int q = 0; q < substeps[i]; q++) c.t[b] += dts[i];
+    }
+  });
+}
 lmw_status lmw_ensemble_steps_timed(lmw_ensemble *e, const double *dts, const int64_t *counts, double *kernel_ms) {
   return guard([&] {
     NEED(e && dts && counts && kernel_ms);
@@ -725,3 +882,28 @@ lmw_status lmw_ensemble_steps_timed(lmw_ensemble *e, const double *dts, const in
 }
 
 }  // extern "C"
+
+namespace lmw {
+// the device state behind the handles, for the corrector's entry points (corrector.hip)
+SphereCore &core_of(lmw_sphere *s) {
+  if (!s) invalid("null pointer argument");
+  return *s->core;
+}
+std::vector<SphereCore *> all_groups(lmw_ensemble *e) {
+  if (!e) invalid("null pointer argument");
+  std::vector<SphereCore *> out;
+  for (auto &g : e->groups) out.push_back(g.get());
+  return out;
+}
+void drop_macro(lmw_ensemble *e) {
+  if (e && e->macro) {
+    LMW_HIP(hipGraphExecDestroy(e->macro));
+    e->macro = nullptr;
+  }
+}
+SphereCore &single_group(lmw_ensemble *e) {
+  if (!e) invalid("null pointer argument");
+  if (e->groups.size() != 1) invalid("the ensemble must hold members of a single truncation");
+  return *e->groups[0];  // one group: local member index == member index
+}
+}  // namespace lmw

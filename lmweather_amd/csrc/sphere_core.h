@@ -27,8 +27,16 @@ struct Geo {
 };
 
 enum AnalMode { ANAL_TEND = 0, ANAL_STATE = 1, ANAL_RAW = 2, ANAL_TENDOUT = 3 };
-enum PrepMode { PREP_STATE = 0, PREP_DIV = 1, PREP_SCALAR = 2 };
+// PREP_TRACER: in3 = U q, V q, q delta -> o1 = D(U q, V q), o4 = the scalar analysis of q delta (sphere.cpp:250-263)
+enum PrepMode { PREP_STATE = 0, PREP_DIV = 1, PREP_SCALAR = 2, PREP_TRACER = 3 };
 enum GridMode { GRID_DIAG = 0, GRID_RAW = 1 };
+
+// Tile variants of the Legendre kernels chosen per truncation group by the autotuner (autotune.hip): FP64
+// members per block; mixed-precision synthesis row chunk; mixed-precision analysis (latitude chunk, waves).
+constexpr int kFp64Tiles = 3, kSynthTiles = 2, kAnalTiles = 6;
+extern const int kFp64MB[kFp64Tiles];
+extern const int kSynthRC[kSynthTiles];
+extern const int kAnalKC[kAnalTiles], kAnalNW[kAnalTiles];
 
 struct SphereCore {
   // geometry
@@ -61,8 +69,15 @@ struct SphereCore {
   DevArray<double> red;                        // [b][16] reductions
   DevArray<int> mflags;                        // [b] bit 0 untilted axis, bit 1 no orography
   DevArray<float2> Fs32, Gb32;                 // FP32 storage of Fs and Gb for the mixed-precision step
+  // passive tracer q (single-member spheres only): state, RK4 stage state and accumulator [S], the spectral input
+  // of its transforms [3][S], the delta grid scratch [4][J][K] and the reference tracer grid [J][K]
+  bool has_tracer = false, advect_only = false;
+  DevArray<double2> q0, qs, qacc, tq;
+  DevArray<double> grid2, ref_q;
   hipGraphExec_t graph = nullptr;
   double graph_dt = 0;
+  // tile variant indices (defaults: FP64 4 members per block, RC 32, KC 64 with 4 waves)
+  int tile_fp64_synth = 1, tile_fp64_anal = 1, tile_synth = 0, tile_anal = 1;
 
   SphereCore(int truncation, const lmw_sphere_params &p, int members, int precision);
   ~SphereCore();
@@ -81,6 +96,13 @@ struct SphereCore {
   void set_spectral_packed(int b, int field, const double2 *packed);  // host packed triangle
   void get_spectral_packed(int b, int field, double2 *packed);
   void mark_reference(int b);
+  // tracer (member 0 of a single-member core): set_tracer_grid (sphere.cpp:241-246), advect-only switch
+  void set_tracer_grid(const double *q);
+  void set_tracer_spectral(const double2 *packed);
+  void set_advect_only(bool on);
+  void tracer_grid(double *out);  // host grid of q
+  double tracer_abs_sum();
+  void drop_graph();
 
   // stepping (all members)
   void launch_stage(int stage, double dt);
@@ -109,6 +131,9 @@ struct SphereCore {
   void launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
   void launch_prep(int mode, int b0, int nb);
   void launch_to_grid(int mode, double *out, int b0, int nb);
+  void launch_tracer_tendency();                     // spec_io[0] = D(U q, V q), spec_io[3] = analysis of q delta
+  void launch_tracer_update(int stage, double dt);   // RK4 stage update of q from spec_io
+  void launch_state_filter(double dt);               // del^4 filter of zeta, delta, phi (advect-only steps)
   void build_mixed_tables();
   void launch_synth_mx(const double2 *spec, int b0, int nb);
   void launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);

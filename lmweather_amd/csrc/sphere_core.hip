@@ -20,8 +20,9 @@
 
 namespace lmw {
 
+const int kFp64MB[kFp64Tiles] = {2, 4, 8};  // members per block of the FP64 Legendre kernels (tile variants)
+
 namespace {
-constexpr int MB = 4;      // members per block in the Legendre kernels
 constexpr int CH = 32;     // latitudes staged per LDS chunk in k_anal
 constexpr int MAXM = 172;  // truncation <= 170 => at most 171 m values
 
@@ -163,8 +164,10 @@ __global__ void k_coriolis(Geo G, const double *mu, double *f, double alpha) {
 
 // ------------------------------------------------------------------------------------------- the step kernels
 
-// Inverse Legendre transform of (zeta, delta, phi) to the Fourier coefficients of U, V, zeta, phi.
-__global__ __launch_bounds__(256) void k_synth(Geo G, const double *__restrict__ Ps, const double *__restrict__ Hs,
+// Inverse Legendre transform of (zeta, delta, phi) to the Fourier coefficients of U, V, zeta, phi.  MB members per
+// block (one per threadIdx.y).
+template <int MB>
+__global__ __launch_bounds__(64 * MB) void k_synth(Geo G, const double *__restrict__ Ps, const double *__restrict__ Hs,
                                                const double2 *__restrict__ spec, double2 *__restrict__ Fs, int b0, int nb) {
   const int m = blockIdx.x, tx = threadIdx.x, ty = threadIdx.y;
   const int jh = blockIdx.y * 64 + tx;
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__r
   const double *g0 = in3 + static_cast<size_t>(blockIdx.y) * 3 * plane;
   for (int i = tid; i < K; i += blockDim.x) tw[i] = twg[i];
   const double c = sqrt(1 - mu[jn] * mu[jn]);
-  const int nrows = mode == PREP_STATE ? 6 : mode == PREP_DIV ? 4 : 2;
+  const int nrows = (mode == PREP_STATE || mode == PREP_TRACER) ? 6 : mode == PREP_DIV ? 4 : 2;
   for (int idx = tid; idx < nrows * L; idx += blockDim.x) {
     const int r = idx / L, k = idx - r * L;
     const int field = r >> 1;
@@ -379,6 +382,12 @@ __global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__r
         const double2 A = rfft_coef<F>(buf + (0 + s) * P, m, tw, invK), B = rfft_coef<F>(buf + (2 + s) * P, m, tw, invK);
         X[s][0] = cscale(cim(dm, A), wd);
         Y[s][0] = cscale(B, -wd);
+      } else if (mode == PREP_TRACER) {  // flux D(A, B) into o1, analyze_scalar(C) into o4
+        const double2 A = rfft_coef<F>(buf + (0 + s) * P, m, tw, invK), B = rfft_coef<F>(buf + (2 + s) * P, m, tw, invK);
+        const double2 Cq = rfft_coef<F>(buf + (4 + s) * P, m, tw, invK);
+        X[s][0] = cscale(cim(dm, A), wd);
+        Y[s][0] = cscale(B, -wd);
+        X[s][3] = cscale(Cq, wj);
       } else {
         X[s][3] = cscale(rfft_coef<F>(buf + s * P, m, tw, invK), wj);
       }
@@ -387,8 +396,9 @@ __global__ __launch_bounds__(256) void k_prep(Geo G, int mode, const double *__r
   }
 }
 
-// Forward Legendre transform and its epilogue (RK4 stage update, state write or raw output).
-__global__ __launch_bounds__(256) void k_anal(Geo G, const double *__restrict__ Pa, const double *__restrict__ Ha,
+// Forward Legendre transform and its epilogue (RK4 stage update, state write or raw output).  MB members per block.
+template <int MB>
+__global__ __launch_bounds__(64 * MB) void k_anal(Geo G, const double *__restrict__ Pa, const double *__restrict__ Ha,
                                               const double2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
                                               double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
                                               double2 *__restrict__ raw, int b0, int nb) {
@@ -573,7 +583,77 @@ __global__ __launch_bounds__(256) void k_abs_sum(const double2 *__restrict__ x0,
   if (threadIdx.x == 0) *out = part[0];
 }
 
+// Tracer grid products (sphere.cpp:255-258): in3 = U q, V q, q delta from raw grids U, V, zeta, q and the delta grid.
+__global__ void k_tracer_products(const double *__restrict__ g4, const double *__restrict__ dgrid, double *__restrict__ in3, size_t plane) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= plane) return;
+  const double U = g4[i], V = g4[plane + i], q = g4[3 * plane + i], d = dgrid[i];
+  in3[i] = U * q;
+  in3[plane + i] = V * q;
+  in3[2 * plane + i] = q * d;
+}
+
+__device__ inline int degree_of(const Geo &G, int i, int &m) {
+  int mm = 0;
+  while (mm + 1 < G.M1 && G.moff[mm + 1] <= i) mm++;
+  m = mm;
+  return i - G.moff[mm] + mm;
+}
+
+// RK4 stage update of the tracer with dq = corr - flux (sphere.cpp:261, :423-433) and the del^4 filter of q at the
+// last stage (:392-405); raw = the analysis outputs (o1 at [0, S), o4 at [3S, 4S)).
+__global__ void k_tracer_rk4(Geo G, int stage, double dt, double nu4, const double2 *__restrict__ raw, double2 *__restrict__ q0,
+                             double2 *__restrict__ qs, double2 *__restrict__ qacc) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.S) return;
+  const double2 k = csub(raw[3 * static_cast<size_t>(G.S) + i], raw[i]);
+  const double2 s0 = q0[i];
+  if (stage == 1) {
+    qacc[i] = k;
+    qs[i] = cadd(s0, cscale(k, 0.5 * dt));
+  } else if (stage == 2) {
+    qacc[i] = cadd(qacc[i], cscale(k, 2.0));
+    qs[i] = cadd(s0, cscale(k, 0.5 * dt));
+  } else if (stage == 3) {
+    qacc[i] = cadd(qacc[i], cscale(k, 2.0));
+    qs[i] = cadd(s0, cscale(k, dt));
+  } else {
+    double2 nx = cadd(s0, cscale(cadd(qacc[i], k), dt / 6));
+    int m;
+    const int n = degree_of(G, i, m);
+    if (nu4 > 0 && n >= 1) {
+      const double kq = n * (n + 1.0) / (G.a * G.a);
+      nx = cscale(nx, exp(-nu4 * kq * kq * dt));
+    }
+    q0[i] = nx;
+    qs[i] = nx;
+  }
+}
+
+// del^4 filter of zeta, delta, phi of member 0 (sphere.cpp:392-405) for advect-only steps, whose frozen dynamics
+// skip the stage kernels that would apply it
+__global__ void k_state_filter(Geo G, double dt, double nu4, double2 *__restrict__ x0, double2 *__restrict__ xs) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.S) return;
+  int m;
+  const int n = degree_of(G, i, m);
+  if (n < 1) return;
+  const double a2 = G.a * G.a, l = n * (n + 1.0);
+  const double kv = (l - 2) / a2, kq = l / a2;
+  const double fv = exp(-nu4 * kv * kv * dt), fp = exp(-nu4 * kq * kq * dt);
+  for (int fld = 0; fld < 3; fld++) {
+    const size_t j = static_cast<size_t>(fld) * G.S + i;
+    const double2 v = cscale(x0[j], fld == 2 ? fp : fv);
+    x0[j] = v;
+    xs[j] = v;
+  }
+}
+
 // ------------------------------------------------------------------------------------------- host side
+
+static void tracer_grid_device(SphereCore &c, double *dst);
 
 SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, int prec) : prm(p), precision(LMW_FP64) {
   if (truncation < 4 || truncation > 170) invalid("truncation must be between 4 and 170");
@@ -704,10 +784,19 @@ void SphereCore::set_orography(int b, const double *hsh) {
   update_flags(b);
 }
 
-void SphereCore::launch_synth(const double2 *spec, int b0, int nb) {
-  dim3 grd(M1, (Jh + 63) / 64, (nb + MB - 1) / MB), blk(64, MB);
-  hipLaunchKernelGGL(k_synth, grd, blk, 0, stream, geo(), Ps.p, Hs.p, spec, Fs.p, b0, nb);
+template <int MB>
+static void synth_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
+  dim3 grd(c.M1, (c.Jh + 63) / 64, (nb + MB - 1) / MB), blk(64, MB);
+  hipLaunchKernelGGL(k_synth<MB>, grd, blk, 0, c.stream, c.geo(), c.Ps.p, c.Hs.p, spec, c.Fs.p, b0, nb);
   LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::launch_synth(const double2 *spec, int b0, int nb) {
+  switch (kFp64MB[tile_fp64_synth]) {
+    case 2: synth_launch<2>(*this, spec, b0, nb); break;
+    case 8: synth_launch<8>(*this, spec, b0, nb); break;
+    default: synth_launch<4>(*this, spec, b0, nb); break;
+  }
 }
 
 // LDS of the grid kernels: 10 padded rows of K/2 plus the K twiddles
@@ -750,10 +839,20 @@ void SphereCore::launch_tend_grid(int b0, int nb, int mixed) {
   LMW_LAUNCH_CHECK();
 }
 
-void SphereCore::launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
-  dim3 grd(M1, (Tmax + 63) / 64, (nb + MB - 1) / MB), blk(64, MB);
-  hipLaunchKernelGGL(k_anal, grd, blk, 0, stream, geo(), Pa.p, Ha.p, Gb.p, mode, stage, dt, prm.diffusion, x0.p, xs.p, acc.p, raw_out, b0, nb);
+template <int MB>
+static void anal_launch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  dim3 grd(c.M1, (c.Tmax + 63) / 64, (nb + MB - 1) / MB), blk(64, MB);
+  hipLaunchKernelGGL(k_anal<MB>, grd, blk, 0, c.stream, c.geo(), c.Pa.p, c.Ha.p, c.Gb.p, mode, stage, dt, c.prm.diffusion, c.x0.p, c.xs.p, c.acc.p,
+                     raw_out, b0, nb);
   LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  switch (kFp64MB[tile_fp64_anal]) {
+    case 2: anal_launch<2>(*this, mode, stage, dt, b0, nb, raw_out); break;
+    case 8: anal_launch<8>(*this, mode, stage, dt, b0, nb, raw_out); break;
+    default: anal_launch<4>(*this, mode, stage, dt, b0, nb, raw_out); break;
+  }
 }
 
 void SphereCore::launch_prep(int mode, int b0, int nb) {
@@ -805,6 +904,7 @@ void SphereCore::mark_reference(int b) {
   launch_to_grid(GRID_RAW, grid.p, b, 1);  // rows of member b land at grid[b][0..3]
   LMW_HIP(hipMemcpyAsync(ref_phi.p + b * plane, grid.p + (static_cast<size_t>(b) * 4 + 3) * plane, plane * sizeof(double),
                          hipMemcpyDeviceToDevice, stream));
+  if (has_tracer) tracer_grid_device(*this, ref_q.p);  // sphere.cpp:238
 }
 
 void SphereCore::launch_stage(int stage, double dt) {
@@ -820,7 +920,113 @@ void SphereCore::launch_stage(int stage, double dt) {
 }
 
 void SphereCore::enqueue_step(double dt) {
-  for (int stage = 1; stage <= 4; stage++) launch_stage(stage, dt);
+  // the tracer tendency reads the stage state before the dynamics' stage kernel advances it (sphere.cpp:412-427)
+  for (int stage = 1; stage <= 4; stage++) {
+    if (has_tracer) launch_tracer_tendency();
+    if (!advect_only) launch_stage(stage, dt);
+    if (has_tracer) launch_tracer_update(stage, dt);
+  }
+  if (advect_only && prm.diffusion > 0) launch_state_filter(dt);
+}
+
+void SphereCore::drop_graph() {
+  if (graph) {
+    LMW_HIP(hipGraphExecDestroy(graph));
+    graph = nullptr;
+  }
+}
+
+void SphereCore::launch_tracer_tendency() {
+  const size_t plane = static_cast<size_t>(J) * K;
+  LMW_HIP(hipMemcpyAsync(tq.p, xs.p, 2 * static_cast<size_t>(S) * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+  LMW_HIP(hipMemcpyAsync(tq.p + 2 * S, qs.p, S * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+  launch_synth(tq.p, 0, 1);
+  launch_to_grid(GRID_RAW, grid.p, 0, 1);  // U, V, zeta, q
+  LMW_HIP(hipMemcpyAsync(tq.p + 2 * S, xs.p + S, S * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+  launch_synth(tq.p, 0, 1);
+  launch_to_grid(GRID_RAW, grid2.p, 0, 1);  // plane 3: delta
+  hipLaunchKernelGGL(k_tracer_products, dim3((plane + 255) / 256), dim3(256), 0, stream, grid.p, grid2.p + 3 * plane, in3.p, plane);
+  LMW_LAUNCH_CHECK();
+  launch_prep(PREP_TRACER, 0, 1);
+  launch_anal(ANAL_RAW, 0, 0.0, 0, 1, spec_io.p);
+}
+
+void SphereCore::launch_tracer_update(int stage, double dt) {
+  hipLaunchKernelGGL(k_tracer_rk4, dim3((S + 255) / 256), dim3(256), 0, stream, geo(), stage, dt, prm.diffusion, spec_io.p, q0.p, qs.p, qacc.p);
+  LMW_LAUNCH_CHECK();
+}
+
+void SphereCore::launch_state_filter(double dt) {
+  hipLaunchKernelGGL(k_state_filter, dim3((S + 255) / 256), dim3(256), 0, stream, geo(), dt, prm.diffusion, x0.p, xs.p);
+  LMW_LAUNCH_CHECK();
+}
+
+static void tracer_buffers(SphereCore &c) {
+  if (c.q0.n) return;
+  const size_t plane = static_cast<size_t>(c.J) * c.K;
+  c.q0.alloc(c.S);
+  c.qs.alloc(c.S);
+  c.qacc.alloc(c.S);
+  c.tq.alloc(3 * static_cast<size_t>(c.S));
+  c.grid2.alloc(4 * plane);
+  c.ref_q.alloc(plane);
+}
+
+// q of member 0 synthesized to the grid at dst (device)
+static void tracer_grid_device(SphereCore &c, double *dst) {
+  const size_t plane = static_cast<size_t>(c.J) * c.K;
+  LMW_HIP(hipMemsetAsync(c.tq.p, 0, 2 * static_cast<size_t>(c.S) * sizeof(double2), c.stream));
+  LMW_HIP(hipMemcpyAsync(c.tq.p + 2 * c.S, c.q0.p, c.S * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
+  c.launch_synth(c.tq.p, 0, 1);
+  c.launch_to_grid(GRID_RAW, c.grid2.p, 0, 1);
+  LMW_HIP(hipMemcpyAsync(dst, c.grid2.p + 3 * plane, plane * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+}
+
+void SphereCore::set_tracer_spectral(const double2 *packed) {
+  if (B != 1) invalid("a tracer needs a single-member sphere");
+  tracer_buffers(*this);
+  q0.upload(packed, S, 0, stream);
+  qs.upload(packed, S, 0, stream);
+  has_tracer = true;
+  drop_graph();
+  sync();
+}
+
+void SphereCore::set_tracer_grid(const double *q) {
+  if (B != 1) invalid("a tracer needs a single-member sphere");
+  tracer_buffers(*this);
+  const size_t plane = static_cast<size_t>(J) * K;
+  in3.upload(q, plane, 0, stream);
+  launch_prep(PREP_SCALAR, 0, 1);
+  launch_anal(ANAL_RAW, 0, 0.0, 0, 1, spec_io.p);
+  LMW_HIP(hipMemcpyAsync(q0.p, spec_io.p + 3 * static_cast<size_t>(S), S * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+  LMW_HIP(hipMemcpyAsync(qs.p, q0.p, S * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+  has_tracer = true;
+  drop_graph();
+  tracer_grid_device(*this, ref_q.p);  // sphere.cpp:245
+  sync();
+}
+
+void SphereCore::set_advect_only(bool on) {
+  if (on != advect_only) drop_graph();
+  advect_only = on;
+}
+
+void SphereCore::tracer_grid(double *out) {
+  if (!has_tracer) runtime("no tracer: set one with tracer() first");
+  const size_t plane = static_cast<size_t>(J) * K;
+  tracer_grid_device(*this, grid2.p);
+  grid2.download(out, plane, 0, stream);
+  sync();
+}
+
+double SphereCore::tracer_abs_sum() {
+  hipLaunchKernelGGL(k_abs_sum, dim3(1), dim3(256), 0, stream, q0.p, static_cast<size_t>(S), red.p + 14);
+  LMW_LAUNCH_CHECK();
+  double s;
+  red.download(&s, 1, 14, stream);
+  sync();
+  return s;
 }
 
 void SphereCore::steps(double dt, int64_t count) {

@@ -103,6 +103,49 @@ class Sphere:
     def mark_reference(self) -> None:
         L.call("lmw_sphere_mark_reference", self._h)
 
+    # ---------------------------------------------------------------- tracer (sphere.hpp:52-64)
+    def set_tracer_grid(self, q) -> None:
+        g = L.as_f64(q).ravel()
+        L.call("lmw_sphere_set_tracer_grid", self._h, L.dptr(g), g.size)
+
+    def set_advect_only(self, on: bool) -> None:
+        L.call("lmw_sphere_set_advect_only", self._h, int(bool(on)))
+
+    def _flags(self):
+        a, b = C.c_int(), C.c_int()
+        L.call("lmw_sphere_tracer_flags", self._h, C.byref(a), C.byref(b))
+        return bool(a.value), bool(b.value)
+
+    def has_tracer(self) -> bool:
+        return self._flags()[0]
+
+    def advect_only(self) -> bool:
+        return self._flags()[1]
+
+    def grid_tracer(self) -> np.ndarray:
+        return self._grid(L.GRID_TRACER)
+
+    def tracer_mean(self) -> float:
+        return self._diag(L.DIAG_TRACER_MEAN)
+
+    def tracer_mass(self) -> float:
+        return self._diag(L.DIAG_TRACER_MASS)
+
+    def tracer_min(self) -> float:
+        return self._diag(L.DIAG_TRACER_MIN)
+
+    def tracer_max(self) -> float:
+        return self._diag(L.DIAG_TRACER_MAX)
+
+    def tracer_error_l2(self) -> float:
+        return self._diag(L.DIAG_TRACER_ERROR_L2)
+
+    def tracer_error_linf(self) -> float:
+        return self._diag(L.DIAG_TRACER_ERROR_LINF)
+
+    def tracer_at(self, lat: float, lon: float) -> float:
+        return self._point(L.POINT_TRACER, lat, lon)
+
     def add_mode(self, n: int, m: int, amplitude: float, mean_depth: float) -> None:
         L.call("lmw_sphere_add_mode", self._h, n, m, amplitude, mean_depth)
 
@@ -317,6 +360,19 @@ class Ensemble:
         L.call("lmw_ensemble_diag", self._h, member, which, C.byref(out))
         return out.value
 
+    def get_truncated(self, truncation: int) -> np.ndarray:
+        """Spectral states of all members (single-truncation ensemble) truncated to ``truncation``: float32 array
+        (members, 3, C, 2) in the packed triangular order, C = (T+1)(T+2)/2."""
+        T = int(truncation)
+        out = np.empty((len(self.truncations), 3, (T + 1) * (T + 2) // 2, 2), dtype=np.float32)
+        L.call("lmw_ensemble_get_truncated", self._h, T, L.fptr(out), out.size)
+        return out
+
+    def set_truncated(self, truncation: int, states) -> None:
+        """Set every member's spectral state from a (members, 3, C_T, 2) float32 array (zero above T)."""
+        a = L.as_f32(states)
+        L.call("lmw_ensemble_set_truncated", self._h, int(truncation), L.fptr(a), a.size)
+
     def steps(self, dts, counts, graph: bool = True) -> None:
         """Advance each truncation group t by counts[t] steps of dts[t] (groups in ascending truncation)."""
         d = np.ascontiguousarray(dts, dtype=np.float64)
@@ -324,6 +380,26 @@ class Ensemble:
         fn = "lmw_ensemble_steps_graph" if graph else "lmw_ensemble_steps"
         L.call(fn, self._h, L.dptr(d), c.ctypes.data_as(L._i64p))
 
+
+    def macro_steps(self, dts, substeps, count: int) -> None:
+        """`count` macro steps; in each, group t advances substeps[t] steps of dts[t] (one HIP graph per macro step,
+        the groups concurrent on their streams)."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        c = np.ascontiguousarray(substeps, dtype=np.int32)
+        L.call("lmw_ensemble_macro_steps", self._h, L.dptr(d), c.ctypes.data_as(L._ip), int(count))
+
+    def autotune(self, force: bool = False):
+        """Choose the Legendre tile shapes of every group (timed, or from the cache); returns (tiles (groups, 4),
+        best_ms (groups, 2))."""
+        ng = len(self.group_truncations)
+        tiles = np.zeros(4 * ng, dtype=np.int32)
+        ms = np.zeros(2 * ng)
+        L.call("lmw_ensemble_autotune", self._h, int(bool(force)), tiles.ctypes.data_as(L._ip), L.dptr(ms))
+        return tiles.reshape(ng, 4), ms.reshape(ng, 2)
+
+    def set_tiles(self, group: int, tiles) -> None:
+        t = np.ascontiguousarray(tiles, dtype=np.int32)
+        L.call("lmw_ensemble_set_tiles", self._h, int(group), t.ctypes.data_as(L._ip))
 
     def steps_timed(self, dts, counts) -> np.ndarray:
         """Eager steps with HIP events around every kernel; returns (groups, 3) total milliseconds of the inverse
@@ -349,6 +425,14 @@ def gauss_legendre(n: int):
     x, w = np.empty(n), np.empty(n)
     L.call("lmw_gauss_legendre", n, L.dptr(x), L.dptr(w))
     return x, w
+
+
+def tile_variant(kind: int, index: int):
+    """Shape of a tile variant: kind 0 FP64 members per block, 1 mixed inverse row chunk, 2 mixed forward
+    (latitude chunk, waves)."""
+    a, b = C.c_int(), C.c_int()
+    L.call("lmw_tile_variants", int(kind), int(index), C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def device_info():

@@ -1,0 +1,97 @@
+"""GPU ensembles of uneven truncations: the graph-captured macro step against per-group stepping, every tile variant
+of the Legendre kernels against the default shapes, and the autotuner (state untouched, cache reused)."""
+import numpy as np
+import pytest
+
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+EARTH = (6.371e6, 7.292e-5, 9.80616)
+
+
+@pytest.fixture(scope="module")
+def lmw():
+    import lmweather_amd
+    return lmweather_amd
+
+
+def build(lmw, truncs, precision):
+    from lmweather_amd.initial import embed, perturb_vorticity, random_balanced_state
+    e = lmw.Ensemble(truncs, lmw.SphereParams(*EARTH), precision)
+    base = lmw.Sphere(42, lmw.SphereParams(*EARTH))
+    h, u, v = random_balanced_state(base, 21, 1)
+    base.set_state_grid(h, u, v)
+    spec = [base.get_spectral(f) for f in range(3)]
+    for i, N in enumerate(truncs):
+        for f in range(3):
+            e.set_spectral(i, f, embed(spec[f], N))
+        e.set_spectral(i, 0, perturb_vorticity(e.get_spectral(i, 0), 1e-3, 1000 + i))
+    return e
+
+
+def states(e):
+    return [np.stack([e.get_spectral(i, f) for f in range(3)]) for i in range(len(e.truncations))]
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_macro_steps_equal_group_steps(lmw, precision):
+    truncs = [42] * 5 + [85] * 3 + [170] * 2
+    a, b = build(lmw, truncs, precision), build(lmw, truncs, precision)
+    a.macro_steps([600.0, 300.0, 150.0], [1, 2, 4], 3)
+    b.steps([600.0, 300.0, 150.0], [3, 6, 12])
+    for x, y in zip(states(a), states(b)):
+        assert np.array_equal(x, y)
+    assert a.diag(9, lmw._lib.DIAG_TIME) == 1800.0 and a.diag(0, lmw._lib.DIAG_TIME) == 1800.0
+
+
+def test_every_tile_variant_matches_default(lmw):
+    """Mixed BF16x3: each (row chunk) x (latitude chunk, waves) variant stays within the BF16x3 error of the default
+    shapes after 4 steps; FP64: the members-per-block variants are bit-identical (members are independent)."""
+    from lmweather_amd.sphere import tile_variant
+    truncs = [85] * 20
+    ref = build(lmw, truncs, 2)
+    ref.steps([300.0], [4])
+    r = states(ref)
+    seen = set()
+    for si in range(2):
+        for ai in range(6):
+            e = build(lmw, truncs, 2)
+            e.set_tiles(0, [1, 1, si, ai])
+            e.steps([300.0], [4])
+            err = max(rel_l2(x[2], y[2]) for x, y in zip(states(e), r))
+            print(f"synth RC {tile_variant(1, si)[0]} anal KC/NW {tile_variant(2, ai)}: rel-L2(phi) vs default {err:.2e}")
+            assert err <= 1e-6
+            seen.add((si, ai))
+    assert len(seen) == 12
+    f_ref = build(lmw, truncs, 0)
+    f_ref.steps([300.0], [2])
+    for v in range(3):
+        e = build(lmw, truncs, 0)
+        e.set_tiles(0, [v, v, 0, 1])
+        e.steps([300.0], [2])
+        for x, y in zip(states(e), states(f_ref)):
+            assert np.array_equal(x, y)
+
+
+def test_autotune_keeps_state_and_uses_cache(lmw, tmp_path, monkeypatch):
+    monkeypatch.setenv("LMW_TUNE_CACHE", str(tmp_path / "tiles.txt"))
+    truncs = [42] * 16 + [170] * 4
+    e = build(lmw, truncs, 2)
+    before = states(e)
+    tiles, ms = e.autotune(force=True)
+    print("tiles", tiles.tolist(), "ms", ms.tolist())
+    assert tiles.shape == (2, 4) and np.all(ms > 0)
+    for x, y in zip(states(e), before):
+        assert np.array_equal(x, y)
+    lines = (tmp_path / "tiles.txt").read_text().split("\n")
+    assert sum(1 for ln in lines if ln.strip()) == 2
+    e2 = build(lmw, truncs, 2)
+    tiles2, ms2 = e2.autotune()
+    assert np.array_equal(tiles2, tiles) and np.all(ms2 == 0)
+    # tuned shapes step to the same answers as the defaults, within BF16x3 error
+    d = build(lmw, truncs, 2)
+    e2.steps([600.0, 150.0], [2, 8])
+    d.steps([600.0, 150.0], [2, 8])
+    for x, y in zip(states(e2), states(d)):
+        assert rel_l2(x[2], y[2]) <= 1e-6
