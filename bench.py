@@ -47,6 +47,7 @@ def build_batch(lmw, members, precision, state):
         ens.set_orography_grid(i, hs)
         ens.set_state_grid(i, h, u, v)
         ens.set_spectral(i, 0, perturb_vorticity(ens.get_spectral(i, 0), 1e-3, 1000 + i))
+    ens.autotune()  # Legendre tile shapes: timed on this batch, or taken from the tile cache
     return ens
 
 
@@ -164,9 +165,96 @@ def plane_bench(lmw, args) -> dict:
     }
 
 
+ENS_GROUPS = ((42, 128, 600.0), (85, 96, 300.0), (170, 32, 150.0))  # truncation, members, dt
+
+
+def ensemble_state(lmw):
+    """The T42 random balanced state (energy up to n = 21, seed 1, no orography): spectral (zeta, delta, phi)."""
+    from lmweather_amd.initial import random_balanced_state
+    base = lmw.Sphere(42, lmw.SphereParams(A, OM, G))
+    h, u, v = random_balanced_state(base, 21, 1)
+    base.set_state_grid(h, u, v)
+    return [base.get_spectral(f) for f in range(3)]
+
+
+def build_uneven(lmw, precision, spec, groups=ENS_GROUPS):
+    """Members in truncation order; member i: the T42 state embedded at its truncation, vorticity perturbed by 1e-3
+    relative noise (seed 1000 + i)."""
+    from lmweather_amd.initial import embed, perturb_vorticity
+    truncs = [t for t, n, _ in groups for _ in range(n)]
+    ens = lmw.Ensemble(truncs, lmw.SphereParams(A, OM, G), precision)
+    for i, t in enumerate(truncs):
+        for f in range(3):
+            ens.set_spectral(i, f, embed(spec[f], t))
+        ens.set_spectral(i, 0, perturb_vorticity(ens.get_spectral(i, 0), 1e-3, 1000 + i))
+    return ens
+
+
+def ensemble_bench(lmw, args) -> dict:
+    """256 members of uneven truncations (128 x T42 at 600 s, 96 x T85 at 300 s, 32 x T170 at 150 s), stepped in
+    600 s macro steps (one HIP graph each: 1, 2 and 4 RK4 steps of the three groups, concurrent on their streams),
+    with autotuned Legendre tiles.  value = member-days/s of the whole ensemble."""
+    prec = {"fp64": lmw.FP64, "fp32": lmw.FP32, "bf16x3": lmw.BF16X3}[args.precision]
+    spec = ensemble_state(lmw)
+    macro = 600.0
+    dts = [dt for _, _, dt in ENS_GROUPS]
+    subs = [int(round(macro / dt)) for dt in dts]
+    nmembers = sum(n for _, n, _ in ENS_GROUPS)
+    days = args.steps * macro / 86400.0
+
+    def run(precision):
+        ens = build_uneven(lmw, precision, spec)
+        tiles, tune_ms = ens.autotune()
+        ens.macro_steps(dts, subs, args.warmup)
+        t0 = time.perf_counter()
+        ens.macro_steps(dts, subs, args.steps)  # synchronous: returns when the device is done
+        wall = time.perf_counter() - t0
+        # per-group kernel times of the same kind of step, eagerly with HIP events (groups one after another)
+        ms = ens.steps_timed(dts, subs)
+        fin = all(np.isfinite(ens.diag(i, lmw._lib.DIAG_ENERGY)) for i in (0, nmembers - 1))
+        del ens
+        return wall, tiles, tune_ms, ms, fin
+
+    print(f"ensemble {args.precision}: {nmembers} members, {args.steps} macro steps of {macro:g} s", file=sys.stderr, flush=True)
+    wall, tiles, tune_ms, ms, fin = run(prec)
+    value = nmembers * days / wall
+    out = {
+        "metric": "ensemble member-days/s, 256 members of uneven truncations (T42/T85/T170)",
+        "value": value,
+        "unit": "member-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 Legendre, f64 state/FFT",
+                  "bf16x3": "bf16x3 Legendre (f32 acc), f64 state/FFT"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "uneven ensemble: 128 x T42 (dt 600 s), 96 x T85 (dt 300 s), 32 x T170 (dt 150 s); "
+                               "T42 balanced random flow n<=21 embedded at each truncation, 1e-3 vorticity perturbations; "
+                               "one step = one 600 s macro step (HIP graph of 1/2/4 RK4 steps per group)",
+                   "members": nmembers, "simulated_days": days, "precision": args.precision},
+        "t170_member_days_per_s": 32 * days / wall,
+        "finite": bool(fin),
+        "tiles": {str(t): {"fp64_mb": lmw.sphere.tile_variant(0, int(tl[0]))[0],
+                           "mixed_synth_rc": lmw.sphere.tile_variant(1, int(tl[2]))[0],
+                           "mixed_anal_kc_nw": list(lmw.sphere.tile_variant(2, int(tl[3])))}
+                  for (t, _, _), tl in zip(ENS_GROUPS, tiles)},
+        "kernel_ms_per_macro_step": {str(t): {"inverse": float(m[0]), "grid": float(m[1]), "forward": float(m[2])}
+                                     for (t, _, _), m in zip(ENS_GROUPS, ms)},
+    }
+    if prec != lmw.FP64 and not args.only_batch:
+        w64, _, _, _, _ = run(lmw.FP64)
+        out["fp64_member_days_per_s"] = nmembers * days / w64
+        out["speedup_vs_fp64_batched"] = value / out["fp64_member_days_per_s"]
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane"])
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=4)
@@ -185,6 +273,9 @@ def main() -> int:
     import lmweather_amd as lmw
     if args.workload == "plane":
         print(json.dumps(plane_bench(lmw, args)))
+        return 0
+    if args.workload == "ensemble":
+        print(json.dumps(ensemble_bench(lmw, args)))
         return 0
     prec = {"fp64": lmw.FP64, "fp32": lmw.FP32, "bf16x3": lmw.BF16X3}[args.precision]
     mixed = prec != lmw.FP64

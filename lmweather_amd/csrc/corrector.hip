@@ -293,25 +293,25 @@ __global__ void k_gather(const float *__restrict__ src, const int *__restrict__ 
   dst[e] = src[static_cast<size_t>(idx[q]) * per + k];
 }
 
-// the state of B members (x [b][3][S] complex, triangle of N1f orders) truncated to T, as float [b][3][C_T][2]
-__global__ void k_truncate_out(const double2 *__restrict__ x, int B, int S, int N1f, int T, float *__restrict__ out) {
-  const int T1 = T + 1, CT = T1 * (T1 + 1) / 2;
+// the state of every member (layout G, triangle of N + 1 orders) truncated to T, as float [b][3][C_T][2]
+__global__ void k_truncate_out(Geo G, const double2 *__restrict__ x, int T, float *__restrict__ out) {
+  const int T1 = T + 1, CT = T1 * (T1 + 1) / 2, N1f = G.N + 1;
   const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= static_cast<size_t>(B) * 3 * CT) return;
+  if (e >= static_cast<size_t>(G.B) * 3 * CT) return;
   const int c = static_cast<int>(e % CT);
   const size_t bf = e / CT;
   int m, n;
   tri_mn(c, T1, m, n);
-  const double2 v = x[bf * S + tri_off(m, N1f) + n - m];
+  const double2 v = x[G.at(static_cast<int>(bf / 3), static_cast<int>(bf % 3), tri_off(m, N1f) + n - m)];
   out[2 * e] = static_cast<float>(v.x);
   out[2 * e + 1] = static_cast<float>(v.y);
 }
 
-// the inverse: every coefficient of B members set from float [b][3][C_T][2] (zero above T), in x0 and xs
-__global__ void k_truncate_in(const float *__restrict__ in, int B, int S, int N1f, int T, double2 *__restrict__ x0, double2 *__restrict__ xs) {
-  const int T1 = T + 1, CT = T1 * (T1 + 1) / 2;
+// the inverse: every coefficient of every member set from float [b][3][C_T][2] (zero above T), in x0 and xs
+__global__ void k_truncate_in(Geo G, const float *__restrict__ in, int T, double2 *__restrict__ x0, double2 *__restrict__ xs) {
+  const int T1 = T + 1, CT = T1 * (T1 + 1) / 2, N1f = G.N + 1, S = G.S;
   const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= static_cast<size_t>(B) * 3 * S) return;
+  if (e >= static_cast<size_t>(G.B) * 3 * S) return;
   const int c = static_cast<int>(e % S);
   const size_t bf = e / S;
   int m, n;
@@ -321,19 +321,35 @@ __global__ void k_truncate_in(const float *__restrict__ in, int B, int S, int N1
     const size_t i = bf * CT + tri_off(m, T1) + n - m;
     v = make_double2(in[2 * i], in[2 * i + 1]);
   }
-  x0[e] = v;
-  xs[e] = v;
+  const size_t o = G.at(static_cast<int>(bf / 3), static_cast<int>(bf % 3), c);
+  x0[o] = v;
+  xs[o] = v;
 }
 
 // x0 += correction (float [b][3][S][2]) and xs = x0, for the members with use[b] != 0
-__global__ void k_add_correction(double2 *__restrict__ x0, double2 *__restrict__ xs, const float *__restrict__ corr, const int *__restrict__ use,
-                                 int B, int S) {
+__global__ void k_add_correction(Geo G, double2 *__restrict__ x0, double2 *__restrict__ xs, const float *__restrict__ corr, const int *__restrict__ use) {
   const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= static_cast<size_t>(B) * 3 * S) return;
-  if (!use[e / (3 * static_cast<size_t>(S))]) return;
-  const double2 v = make_double2(x0[e].x + corr[2 * e], x0[e].y + corr[2 * e + 1]);
-  x0[e] = v;
-  xs[e] = v;
+  const size_t S = G.S;
+  if (e >= static_cast<size_t>(G.B) * 3 * S) return;
+  const int b = static_cast<int>(e / (3 * S)), f = static_cast<int>((e / S) % 3);
+  if (!use[b]) return;
+  const size_t o = G.at(b, f, e % S);
+  const double2 v = make_double2(x0[o].x + corr[2 * e], x0[o].y + corr[2 * e + 1]);
+  x0[o] = v;
+  xs[o] = v;
+}
+
+// x0 = xs = kept for the members with use[b] != 0 (every array in layout G)
+__global__ void k_restore_members(Geo G, const double2 *__restrict__ kept, const int *__restrict__ use, double2 *__restrict__ x0,
+                                  double2 *__restrict__ xs) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t S = G.S;
+  if (e >= static_cast<size_t>(G.B) * 3 * S) return;
+  const int b = static_cast<int>(e / (3 * S));
+  if (!use[b]) return;
+  const size_t o = G.at(b, static_cast<int>((e / S) % 3), e % S);
+  x0[o] = kept[o];
+  xs[o] = kept[o];
 }
 
 unsigned blocks(size_t n, unsigned t = 256) { return static_cast<unsigned>((n + t - 1) / t); }
@@ -591,7 +607,7 @@ void corrected_step(SphereCore &core, Corrector &k, double dt, int nc, int nf, d
   LMW_HIP(hipMemcpyAsync(save.p, core.x0.p, nstate * sizeof(double2), hipMemcpyDeviceToDevice, core.stream));
   core.steps(dt / nc, nc);
   k.ensure(B);
-  hipLaunchKernelGGL(k_truncate_out, dim3(blocks(nstate)), dim3(256), 0, core.stream, core.x0.p, B, S, core.N + 1, core.N, k.raw.p);
+  hipLaunchKernelGGL(k_truncate_out, dim3(blocks(nstate)), dim3(256), 0, core.stream, core.geo(), core.x0.p, core.N, k.raw.p);
   LMW_LAUNCH_CHECK();
   core.sync();
   std::vector<double> res(B);
@@ -607,7 +623,7 @@ void corrected_step(SphereCore &core, Corrector &k, double dt, int nc, int nf, d
   DevArray<int> duse;
   duse.alloc(B);
   duse.upload(use.data(), B, 0, core.stream);
-  hipLaunchKernelGGL(k_add_correction, dim3(blocks(nstate)), dim3(256), 0, core.stream, core.x0.p, core.xs.p, k.raw2.p, duse.p, B, S);
+  hipLaunchKernelGGL(k_add_correction, dim3(blocks(nstate)), dim3(256), 0, core.stream, core.geo(), core.x0.p, core.xs.p, k.raw2.p, duse.p);
   LMW_LAUNCH_CHECK();
   if (nfall > 0) {
     // the corrected members are kept aside, every member restarts from the saved state with the plain sub-steps,
@@ -618,12 +634,8 @@ void corrected_step(SphereCore &core, Corrector &k, double dt, int nc, int nf, d
     LMW_HIP(hipMemcpyAsync(core.x0.p, save.p, nstate * sizeof(double2), hipMemcpyDeviceToDevice, core.stream));
     LMW_HIP(hipMemcpyAsync(core.xs.p, save.p, nstate * sizeof(double2), hipMemcpyDeviceToDevice, core.stream));
     core.steps(dt / nf, nf);
-    const size_t per = static_cast<size_t>(3) * S;
-    for (int b = 0; b < B; b++)
-      if (use[b]) {
-        LMW_HIP(hipMemcpyAsync(core.x0.p + b * per, kept.p + b * per, per * sizeof(double2), hipMemcpyDeviceToDevice, core.stream));
-        LMW_HIP(hipMemcpyAsync(core.xs.p + b * per, kept.p + b * per, per * sizeof(double2), hipMemcpyDeviceToDevice, core.stream));
-      }
+    hipLaunchKernelGGL(k_restore_members, dim3(blocks(nstate)), dim3(256), 0, core.stream, core.geo(), kept.p, duse.p, core.x0.p, core.xs.p);
+    LMW_LAUNCH_CHECK();
     core.sync();
   }
   core.sync();
@@ -636,7 +648,7 @@ void truncated_out(SphereCore &c, int T, float *out, size_t n) {
   if (n != need) invalid("buffer size mismatch");
   DevArray<float> d;
   d.alloc(need);
-  hipLaunchKernelGGL(k_truncate_out, dim3(blocks(need / 2)), dim3(256), 0, c.stream, c.x0.p, c.B, c.S, c.N + 1, T, d.p);
+  hipLaunchKernelGGL(k_truncate_out, dim3(blocks(need / 2)), dim3(256), 0, c.stream, c.geo(), c.x0.p, T, d.p);
   LMW_LAUNCH_CHECK();
   d.download(out, need, 0, c.stream);
   c.sync();
@@ -648,8 +660,7 @@ void truncated_in(SphereCore &c, int T, const float *in, size_t n) {
   DevArray<float> d;
   d.alloc(need);
   d.upload(in, need, 0, c.stream);
-  hipLaunchKernelGGL(k_truncate_in, dim3(blocks(static_cast<size_t>(c.B) * 3 * c.S)), dim3(256), 0, c.stream, d.p, c.B, c.S, c.N + 1, T, c.x0.p,
-                     c.xs.p);
+  hipLaunchKernelGGL(k_truncate_in, dim3(blocks(static_cast<size_t>(c.B) * 3 * c.S)), dim3(256), 0, c.stream, c.geo(), d.p, T, c.x0.p, c.xs.p);
   LMW_LAUNCH_CHECK();
   c.sync();
 }

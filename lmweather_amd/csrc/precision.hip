@@ -141,10 +141,10 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
           const int kq = c0 + kk + h, k = 2 * kq + p, n = m + k;
           double2 z = cx(0, 0), d = cx(0, 0), f = cx(0, 0);
           if (bl < nb && k < Km && kq < kp) {
-            const size_t off = static_cast<size_t>(b0 + bl) * 3 * G.S + G.moff[m] + k;
-            z = spec[off];
-            d = spec[off + G.S];
-            f = spec[off + 2 * G.S];
+            const size_t q = static_cast<size_t>(G.moff[m]) + k;
+            z = spec[G.at(b0 + bl, 0, q)];
+            d = spec[G.at(b0 + bl, 1, q)];
+            f = spec[G.at(b0 + bl, 2, q)];
             if (mixed && n == 0) f = cx(0, 0);  // phi_00 P_0^0 is added in FP64 below
           }
           const double kn = n >= 1 ? -a * a / (n * (n + 1.0)) : 0.0;  // psi, chi = kn (zeta, delta): sphere.cpp:185-190
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
   if (bl >= nb) return;
   const int b = b0 + bl;
   double phi00 = 0;
-  if (mixed && m == 0) phi00 = spec[static_cast<size_t>(b) * 3 * G.S + 2 * G.S].x * 0.70710678118654752440;
+  if (mixed && m == 0) phi00 = spec[G.at(b, 2, 0)].x * 0.70710678118654752440;
   const double dm = m;
   for (int r = 0; r < 4; r++) {
     const int jh = jt * 16 + 4 * (lane >> 4) + r;

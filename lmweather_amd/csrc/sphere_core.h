@@ -24,6 +24,14 @@ struct Geo {
   int N, M1, J, K, logK, Jh, Jhp, B, S, Tmax;
   double a, g, omega;
   const int *moff, *poff, *kpe, *kpo;
+  // element strides of the spectral state arrays x0/xs/acc: entry (member b, field f, packed q) at
+  // b sb + f sf + q sq.  Member-major [b][f][q] (sb = 3S, sf = S, sq = 1) for FP64 groups; member-minor [f][q][b]
+  // (sb = 1, sf = S B, sq = B) for mixed-precision groups of several members, whose Legendre kernels map lanes to
+  // members and so read and write 16 consecutive members as one contiguous 256-byte segment.
+  long long sb, sf, sq;
+  __host__ __device__ size_t at(int b, int f, size_t q) const {
+    return static_cast<size_t>(b * sb + f * sf + static_cast<long long>(q) * sq);
+  }
 };
 
 enum AnalMode { ANAL_TEND = 0, ANAL_STATE = 1, ANAL_RAW = 2, ANAL_TENDOUT = 3 };
@@ -78,6 +86,7 @@ struct SphereCore {
   double graph_dt = 0;
   // tile variant indices (defaults: FP64 4 members per block, RC 32, KC 64 with 4 waves)
   int tile_fp64_synth = 1, tile_fp64_anal = 1, tile_synth = 0, tile_anal = 1;
+  bool member_minor = false;  // layout of x0/xs/acc (see Geo)
 
   SphereCore(int truncation, const lmw_sphere_params &p, int members, int precision);
   ~SphereCore();
@@ -126,7 +135,8 @@ struct SphereCore {
   void tendencies_raw(int b, double2 *dz, double2 *dd, double2 *df);
 
   // launches
-  void launch_synth(const double2 *spec, int b0, int nb);
+  // scratch: spec is a contiguous [member][3][S] buffer (spec_io, tq), not the state arrays
+  void launch_synth(const double2 *spec, int b0, int nb, bool scratch = false);
   void launch_tend_grid(int b0, int nb, int mixed = 0);
   void launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
   void launch_prep(int mode, int b0, int nb);

@@ -177,15 +177,12 @@ __global__ __launch_bounds__(64 * MB) void k_synth(Geo G, const double *__restri
   const double a = G.a;
   if (bl < nb) {
     const size_t off = static_cast<size_t>(G.moff[m]);
-    const double2 *sz = spec + (static_cast<size_t>(b) * 3 + 0) * G.S + off;
-    const double2 *sd = spec + (static_cast<size_t>(b) * 3 + 1) * G.S + off;
-    const double2 *sf = spec + (static_cast<size_t>(b) * 3 + 2) * G.S + off;
     for (int k = tx; k < Km; k += 64) {
       const int n = m + k;
       const double kn = n >= 1 ? -a * a / (n * (n + 1.0)) : 0.0;  // psi, chi = -a^2/(n(n+1)) (zeta, delta): :185-190
-      const double2 z = sz[k], d = sd[k];
+      const double2 z = spec[G.at(b, 0, off + k)], d = spec[G.at(b, 1, off + k)];
       cz[ty][k] = z;
-      cf[ty][k] = sf[k];
+      cf[ty][k] = spec[G.at(b, 2, off + k)];
       cp[ty][k] = cscale(z, kn);
       cc[ty][k] = cscale(d, kn);
     }
@@ -253,7 +250,7 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   double2 *buf = lds, *tw = lds + 10 * P;
   // mixed precision: the phi fluxes and the Bernoulli function are formed with phi - phibar (the area mean
   // phi_00 / sqrt 2), whose exact contribution -phibar delta the forward transform's epilogue adds back
-  const double phibar = mixed ? xs[static_cast<size_t>(b) * 3 * G.S + 2 * G.S].x * 0.70710678118654752440 : 0.0;
+  const double phibar = mixed ? xs[G.at(b, 2, 0)].x * 0.70710678118654752440 : 0.0;
   for (int i = tid; i < K; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
   for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
@@ -651,6 +648,47 @@ __global__ void k_state_filter(Geo G, double dt, double nu4, double2 *__restrict
   }
 }
 
+// sum of |coefficient| of member b of a strided state array (check_finite)
+__global__ __launch_bounds__(256) void k_abs_sum_member(Geo G, int b, const double2 *__restrict__ x, double *__restrict__ out) {
+  __shared__ double part[256];
+  double s = 0;
+  const size_t n = 3 * static_cast<size_t>(G.S);
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double2 v = x[G.at(b, static_cast<int>(i / G.S), i % G.S)];
+    s += hypot(v.x, v.y);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = part[0];
+}
+
+// member b, field f of a strided state array <-> a contiguous [S] buffer
+__global__ void k_member_field(Geo G, int b, int f, double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ buf, int to_state) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= G.S) return;
+  const size_t i = G.at(b, f, q);
+  if (to_state) {
+    x0[i] = buf[q];
+    xs[i] = buf[q];
+  } else {
+    buf[q] = x0[i];
+  }
+}
+
+// dst (layout Gd) <- src (layout Gs), every member, field and coefficient
+__global__ void k_relayout(Geo Gs, Geo Gd, const double2 *__restrict__ src, double2 *__restrict__ dst) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t n3 = 3 * static_cast<size_t>(Gs.S);
+  if (e >= n3 * Gs.B) return;
+  const int b = static_cast<int>(e / n3), f = static_cast<int>((e % n3) / Gs.S);
+  const size_t q = e % Gs.S;
+  dst[Gd.at(b, f, q)] = src[Gs.at(b, f, q)];
+}
+
 // ------------------------------------------------------------------------------------------- host side
 
 static void tracer_grid_device(SphereCore &c, double *dst);
@@ -751,6 +789,15 @@ Geo SphereCore::geo() const {
   G.B = B;
   G.S = S;
   G.Tmax = Tmax;
+  if (member_minor) {
+    G.sb = 1;
+    G.sf = static_cast<long long>(S) * B;
+    G.sq = B;
+  } else {
+    G.sb = 3LL * S;
+    G.sf = S;
+    G.sq = 1;
+  }
   G.a = prm.radius;
   G.g = prm.gravity;
   G.omega = prm.omega;
@@ -785,17 +832,23 @@ void SphereCore::set_orography(int b, const double *hsh) {
 }
 
 template <int MB>
-static void synth_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
+static void synth_launch(SphereCore &c, const double2 *spec, int b0, int nb, bool scratch) {
+  Geo G = c.geo();
+  if (scratch) {  // member-major scratch buffers
+    G.sb = 3LL * c.S;
+    G.sf = c.S;
+    G.sq = 1;
+  }
   dim3 grd(c.M1, (c.Jh + 63) / 64, (nb + MB - 1) / MB), blk(64, MB);
-  hipLaunchKernelGGL(k_synth<MB>, grd, blk, 0, c.stream, c.geo(), c.Ps.p, c.Hs.p, spec, c.Fs.p, b0, nb);
+  hipLaunchKernelGGL(k_synth<MB>, grd, blk, 0, c.stream, G, c.Ps.p, c.Hs.p, spec, c.Fs.p, b0, nb);
   LMW_LAUNCH_CHECK();
 }
 
-void SphereCore::launch_synth(const double2 *spec, int b0, int nb) {
+void SphereCore::launch_synth(const double2 *spec, int b0, int nb, bool scratch) {
   switch (kFp64MB[tile_fp64_synth]) {
-    case 2: synth_launch<2>(*this, spec, b0, nb); break;
-    case 8: synth_launch<8>(*this, spec, b0, nb); break;
-    default: synth_launch<4>(*this, spec, b0, nb); break;
+    case 2: synth_launch<2>(*this, spec, b0, nb, scratch); break;
+    case 8: synth_launch<8>(*this, spec, b0, nb, scratch); break;
+    default: synth_launch<4>(*this, spec, b0, nb, scratch); break;
   }
 }
 
@@ -869,6 +922,21 @@ void SphereCore::launch_to_grid(int mode, double *out, int b0, int nb) {
 void SphereCore::set_precision(int p) {
   if (p != LMW_FP64 && p != LMW_FP32 && p != LMW_BF16X3) invalid("precision must be LMW_FP64, LMW_FP32 or LMW_BF16X3");
   if (p != LMW_FP64) build_mixed_tables();
+  const bool minor = p != LMW_FP64 && B > 1;
+  if (minor != member_minor) {  // re-lay the state for the new precision's Legendre kernels
+    const Geo Gs = geo();
+    member_minor = minor;
+    const Geo Gd = geo();
+    const size_t n = x0.n;
+    DevArray<double2> tmp;
+    tmp.alloc(n);
+    for (DevArray<double2> *a : {&x0, &xs, &acc}) {
+      hipLaunchKernelGGL(k_relayout, dim3((n + 255) / 256), dim3(256), 0, stream, Gs, Gd, a->p, tmp.p);
+      LMW_LAUNCH_CHECK();
+      LMW_HIP(hipMemcpyAsync(a->p, tmp.p, n * sizeof(double2), hipMemcpyDeviceToDevice, stream));
+    }
+    sync();
+  }
   precision = p;
   if (graph) {
     LMW_HIP(hipGraphExecDestroy(graph));
@@ -888,12 +956,26 @@ void SphereCore::set_state_grid(int b, const double *h, const double *u, const d
 }
 
 void SphereCore::set_spectral_packed(int b, int field, const double2 *packed) {
+  if (member_minor) {
+    spec_io.upload(packed, S, 0, stream);
+    hipLaunchKernelGGL(k_member_field, dim3((S + 255) / 256), dim3(256), 0, stream, geo(), b, field, x0.p, xs.p, spec_io.p, 1);
+    LMW_LAUNCH_CHECK();
+    sync();
+    return;
+  }
   const size_t off = (static_cast<size_t>(b) * 3 + field) * S;
   x0.upload(packed, S, off, stream);
   xs.upload(packed, S, off, stream);
 }
 
 void SphereCore::get_spectral_packed(int b, int field, double2 *packed) {
+  if (member_minor) {
+    hipLaunchKernelGGL(k_member_field, dim3((S + 255) / 256), dim3(256), 0, stream, geo(), b, field, x0.p, xs.p, spec_io.p, 0);
+    LMW_LAUNCH_CHECK();
+    spec_io.download(packed, S, 0, stream);
+    sync();
+    return;
+  }
   x0.download(packed, S, (static_cast<size_t>(b) * 3 + field) * S, stream);
   sync();
 }
@@ -940,10 +1022,10 @@ void SphereCore::launch_tracer_tendency() {
   const size_t plane = static_cast<size_t>(J) * K;
   LMW_HIP(hipMemcpyAsync(tq.p, xs.p, 2 * static_cast<size_t>(S) * sizeof(double2), hipMemcpyDeviceToDevice, stream));
   LMW_HIP(hipMemcpyAsync(tq.p + 2 * S, qs.p, S * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-  launch_synth(tq.p, 0, 1);
+  launch_synth(tq.p, 0, 1, true);
   launch_to_grid(GRID_RAW, grid.p, 0, 1);  // U, V, zeta, q
   LMW_HIP(hipMemcpyAsync(tq.p + 2 * S, xs.p + S, S * sizeof(double2), hipMemcpyDeviceToDevice, stream));
-  launch_synth(tq.p, 0, 1);
+  launch_synth(tq.p, 0, 1, true);
   launch_to_grid(GRID_RAW, grid2.p, 0, 1);  // plane 3: delta
   hipLaunchKernelGGL(k_tracer_products, dim3((plane + 255) / 256), dim3(256), 0, stream, grid.p, grid2.p + 3 * plane, in3.p, plane);
   LMW_LAUNCH_CHECK();
@@ -977,7 +1059,7 @@ static void tracer_grid_device(SphereCore &c, double *dst) {
   const size_t plane = static_cast<size_t>(c.J) * c.K;
   LMW_HIP(hipMemsetAsync(c.tq.p, 0, 2 * static_cast<size_t>(c.S) * sizeof(double2), c.stream));
   LMW_HIP(hipMemcpyAsync(c.tq.p + 2 * c.S, c.q0.p, c.S * sizeof(double2), hipMemcpyDeviceToDevice, c.stream));
-  c.launch_synth(c.tq.p, 0, 1);
+  c.launch_synth(c.tq.p, 0, 1, true);
   c.launch_to_grid(GRID_RAW, c.grid2.p, 0, 1);
   LMW_HIP(hipMemcpyAsync(dst, c.grid2.p + 3 * plane, plane * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
 }
@@ -1101,8 +1183,7 @@ void SphereCore::download_red(int b, double *out16) {
 }
 
 double SphereCore::spectral_abs_sum(int b) {
-  hipLaunchKernelGGL(k_abs_sum, dim3(1), dim3(256), 0, stream, x0.p + static_cast<size_t>(b) * 3 * S, static_cast<size_t>(3) * S,
-                     red.p + static_cast<size_t>(b) * 16 + 15);
+  hipLaunchKernelGGL(k_abs_sum_member, dim3(1), dim3(256), 0, stream, geo(), b, x0.p, red.p + static_cast<size_t>(b) * 16 + 15);
   LMW_LAUNCH_CHECK();
   double s;
   red.download(&s, 1, static_cast<size_t>(b) * 16 + 15, stream);
@@ -1116,7 +1197,7 @@ void SphereCore::synthesize_raw(const double2 *z, const double2 *d, const double
   spec_io.upload(z ? z : zero.data(), S, 0, stream);
   spec_io.upload(d ? d : zero.data(), S, S, stream);
   spec_io.upload(phi ? phi : zero.data(), S, 2 * S, stream);
-  launch_synth(spec_io.p, 0, 1);
+  launch_synth(spec_io.p, 0, 1, true);
   launch_to_grid(GRID_RAW, grid.p, 0, 1);
   const size_t plane = static_cast<size_t>(J) * K;
   if (U) grid.download(U, plane, 0, stream);

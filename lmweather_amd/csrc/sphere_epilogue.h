@@ -23,20 +23,18 @@ __device__ inline void anal_epilogue(const Geo &G, int mode, int stage, double d
   }
   if (mode == ANAL_STATE) {
     if (q == 0) o1 = o2 = cx(0, 0);  // the constant is not defined by the winds (sphere.cpp:213-214)
-    const size_t i0 = static_cast<size_t>(b) * 3 * S + q;
-    x0[i0] = o1;
-    x0[i0 + S] = o2;
-    x0[i0 + 2 * S] = o4;
-    xs[i0] = o1;
-    xs[i0 + S] = o2;
-    xs[i0 + 2 * S] = o4;
+    x0[G.at(b, 0, q)] = o1;
+    x0[G.at(b, 1, q)] = o2;
+    x0[G.at(b, 2, q)] = o4;
+    xs[G.at(b, 0, q)] = o1;
+    xs[G.at(b, 1, q)] = o2;
+    xs[G.at(b, 2, q)] = o4;
     return;
   }
   double2 kz = o1, kd = cadd(o2, cscale(o3, n * (n + 1.0) / (G.a * G.a))), kf = o4;
   if (mixed) {
-    const size_t i0 = static_cast<size_t>(b) * 3 * S;
-    const double phibar = xs[i0 + 2 * S].x * 0.70710678118654752440;  // phi_00 P_0^0
-    kf = csub(kf, cscale(xs[i0 + S + q], phibar));
+    const double phibar = xs[G.at(b, 2, 0)].x * 0.70710678118654752440;  // phi_00 P_0^0
+    kf = csub(kf, cscale(xs[G.at(b, 1, q)], phibar));
   }
   if (q == 0) kz = kd = kf = cx(0, 0);  // sphere.cpp:387-389
   if (mode == ANAL_TENDOUT) {           // tendencies only, indexed by the local member
@@ -56,7 +54,7 @@ __device__ inline void anal_epilogue(const Geo &G, int mode, int stage, double d
       fpp = exp(-nu4 * kq * kq * dt);
     }
     for (int fld = 0; fld < 3; fld++) {
-      const size_t i = (static_cast<size_t>(b) * 3 + fld) * S + q;
+      const size_t i = G.at(b, fld, q);
       const double2 s0 = x0[i], kv = kk[fld];
       if (stage == 1) {
         acc[i] = kv;
