@@ -27,7 +27,11 @@ sys.path.insert(0, ROOT)
 
 A, OM, G = 6.371e6, 7.292e-5, 9.80616
 N, DT = 170, 150.0
-PEAKS = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
+PEAKS = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15, "bf16x6": 2.5e15}
+MFMA_PRODUCTS = {"fp64": 1.0, "fp32": 1.0, "bf16x3": 3.0, "bf16x6": 6.0}  # MFMA products per multiply-add
+PREC_NAMES = ("fp64", "fp32", "bf16x3", "bf16x6")
+DTYPES = {"fp64": "f64", "fp32": "f32 Legendre, f64 state/FFT", "bf16x3": "bf16x3 Legendre (f32 acc), f64 state/FFT",
+          "bf16x6": "bf16x6 Legendre (f32 acc), f64 state/FFT"}
 HBM_PEAK = 8.0e12
 
 
@@ -194,7 +198,7 @@ def ensemble_bench(lmw, args) -> dict:
     """256 members of uneven truncations (128 x T42 at 600 s, 96 x T85 at 300 s, 32 x T170 at 150 s), stepped in
     600 s macro steps (one HIP graph each: 1, 2 and 4 RK4 steps of the three groups, concurrent on their streams),
     with autotuned Legendre tiles.  value = member-days/s of the whole ensemble."""
-    prec = {"fp64": lmw.FP64, "fp32": lmw.FP32, "bf16x3": lmw.BF16X3}[args.precision]
+    prec = PREC_NAMES.index(args.precision)
     spec = ensemble_state(lmw)
     macro = 600.0
     dts = [dt for _, _, dt in ENS_GROUPS]
@@ -229,8 +233,7 @@ def ensemble_bench(lmw, args) -> dict:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 Legendre, f64 state/FFT",
-                  "bf16x3": "bf16x3 Legendre (f32 acc), f64 state/FFT"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": "synthetic",
         "config": {"workload": "uneven ensemble: 128 x T42 (dt 600 s), 96 x T85 (dt 300 s), 32 x T170 (dt 150 s); "
                                "T42 balanced random flow n<=21 embedded at each truncation, 1e-3 vorticity perturbations; "
@@ -259,7 +262,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--members", type=int, default=32)
-    ap.add_argument("--precision", default="bf16x3", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--precision", default="bf16x6", choices=list(PREC_NAMES))
     ap.add_argument("--check-days", type=float, default=1.0)
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
@@ -277,7 +280,7 @@ def main() -> int:
     if args.workload == "ensemble":
         print(json.dumps(ensemble_bench(lmw, args)))
         return 0
-    prec = {"fp64": lmw.FP64, "fp32": lmw.FP32, "bf16x3": lmw.BF16X3}[args.precision]
+    prec = PREC_NAMES.index(args.precision)
     mixed = prec != lmw.FP64
     name, cus, _ = lmw.device_info()
     print(f"device {name} ({cus} CUs); building {args.members} T170 members", file=sys.stderr, flush=True)
@@ -294,7 +297,7 @@ def main() -> int:
     synth_f, grid_b, anal_f = kernel_costs(J, K, S, N + 1, args.members, mixed)
     avg = ms / launches * 1e-3  # seconds per launch
     peak = PEAKS[args.precision]
-    mult = 3.0 if args.precision == "bf16x3" else 1.0  # MFMA products per multiply-add
+    mult = MFMA_PRODUCTS[args.precision]
     kernels = {
         "legendre_inverse": {"kernel": "k_synth_mx" if mixed else "k_synth", "avg_us": avg[0] * 1e6,
                              "tflops": synth_f / avg[0] / 1e12, "frac": mult * synth_f / avg[0] / peak},
@@ -325,8 +328,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32 Legendre, f64 state/FFT",
-                  "bf16x3": "bf16x3 Legendre (f32 acc), f64 state/FFT"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": "synthetic",
         "config": {"workload": f"T170 ensemble, {args.members} members (512x256 Gaussian grid, dt {DT:g} s, RK4), "
                                "balanced random flow n<=85 over 8 Gaussian mountains, 1e-3 vorticity perturbations",

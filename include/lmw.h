@@ -38,6 +38,7 @@ typedef int lmw_status;
 #define LMW_FP64 0
 #define LMW_FP32 1   /* FP32 tables and operands, FP32 accumulation */
 #define LMW_BF16X3 2 /* split BF16 hi+lo operands, three MFMA products, FP32 accumulation */
+#define LMW_BF16X6 3 /* operands split in three BF16 parts (FP32-exact), six MFMA products, FP32 accumulation */
 
 /* grid fields of lmw_sphere_get_grid (reference grid_height/grid_u/grid_v/grid_vorticity/grid_tracer,
  * cpp/sphere.hpp:91-95,55) */

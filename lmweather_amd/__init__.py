@@ -4,6 +4,6 @@ The compute path is liblmw_hip.so (hand-written HIP kernels, C ABI in include/lm
 mirror of the reference classes lmweather::Sphere and lmweather::Plane over that ABI, plus ensembles, the
 initial-condition builders of the benchmark configurations and the learned coarse-step corrector.
 """
-from ._lib import (BF16X3, FP32, FP64, DeviceError, PlaneParams, SphereParams)  # noqa: F401
+from ._lib import (BF16X3, BF16X6, FP32, FP64, DeviceError, PlaneParams, SphereParams)  # noqa: F401
 from .sphere import Ensemble, Sphere, device_info, fft, gauss_legendre  # noqa: F401
 from .plane import Plane  # noqa: F401

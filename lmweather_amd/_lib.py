@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblmw_hip.so")
 
 LMW_OK, LMW_EINVAL, LMW_ERUNTIME, LMW_EDEVICE = 0, 1, 2, 3
-FP64, FP32, BF16X3 = 0, 1, 2
+FP64, FP32, BF16X3, BF16X6 = 0, 1, 2, 3
 GRID_H, GRID_U, GRID_V, GRID_VORTICITY, GRID_TRACER = 0, 1, 2, 3, 4
 SPEC_ZETA, SPEC_DELTA, SPEC_PHI, SPEC_Q = 0, 1, 2, 3
 (DIAG_MEAN_HEIGHT, DIAG_ENERGY, DIAG_KINETIC, DIAG_POTENTIAL, DIAG_ENSTROPHY, DIAG_MAX_WIND, DIAG_ERROR_L2, DIAG_ERROR_LINF,

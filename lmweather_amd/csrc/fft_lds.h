@@ -44,6 +44,47 @@ __device__ inline double2 w16(int k) {
 }
 }  // namespace fftd
 
+// v * W_16^K (SIGN < 0) or v * conj(W_16^K) (SIGN > 0), with the multiplications by 1, -i, +i and (1 -+ i)/sqrt 2
+// done as swaps, negations and one scaled sum/difference instead of a full complex product
+template <int K, int SIGN> __device__ inline double2 mul_w16(double2 v) {
+  constexpr double h = 0.7071067811865476;
+  constexpr int k = K & 7;
+  if constexpr (k == 0) {
+    return v;
+  } else if constexpr (k == 4) {
+    return SIGN < 0 ? make_double2(v.y, -v.x) : make_double2(-v.y, v.x);  // * -i, * +i
+  } else if constexpr (k == 2) {
+    return SIGN < 0 ? make_double2(h * (v.x + v.y), h * (v.y - v.x)) : make_double2(h * (v.x - v.y), h * (v.x + v.y));
+  } else if constexpr (k == 6) {
+    return SIGN < 0 ? make_double2(h * (v.y - v.x), -h * (v.x + v.y)) : make_double2(-h * (v.x + v.y), h * (v.x - v.y));
+  } else {
+    double2 w = fftd::w16(k);
+    if (SIGN > 0) w.y = -w.y;
+    return cmul(v, w);
+  }
+}
+
+template <int R, int SIGN, int LEN, int I, int KK> __device__ inline void butterflies(double2 (&v)[R]) {
+  if constexpr (KK < LEN / 2) {
+    const double2 t = mul_w16<KK * (16 / LEN), SIGN>(v[I + KK + LEN / 2]);
+    v[I + KK + LEN / 2] = csub(v[I + KK], t);
+    v[I + KK] = cadd(v[I + KK], t);
+    butterflies<R, SIGN, LEN, I, KK + 1>(v);
+  }
+}
+template <int R, int SIGN, int LEN, int I> __device__ inline void stage(double2 (&v)[R]) {
+  if constexpr (I < R) {
+    butterflies<R, SIGN, LEN, I, 0>(v);
+    stage<R, SIGN, LEN, I + LEN>(v);
+  }
+}
+template <int R, int SIGN, int LEN> __device__ inline void stages(double2 (&v)[R]) {
+  if constexpr (LEN <= R) {
+    stage<R, SIGN, LEN, 0>(v);
+    stages<R, SIGN, 2 * LEN>(v);
+  }
+}
+
 // In-register DFT of length R (<= 16): v_k <- sum_n v_n exp(SIGN 2 pi i n k / R), radix-2 decimation in time.
 template <int R, int SIGN> __device__ inline void dft_reg(double2 (&v)[R]) {
 #pragma unroll
@@ -55,20 +96,7 @@ template <int R, int SIGN> __device__ inline void dft_reg(double2 (&v)[R]) {
       v[j] = t;
     }
   }
-#pragma unroll
-  for (int len = 2; len <= R; len <<= 1) {
-#pragma unroll
-    for (int i = 0; i < R; i += len) {
-#pragma unroll
-      for (int k = 0; k < len / 2; k++) {
-        double2 w = fftd::w16(k * (16 / len));
-        if (SIGN > 0) w.y = -w.y;
-        const double2 t = cmul(v[i + k + len / 2], w);
-        v[i + k + len / 2] = csub(v[i + k], t);
-        v[i + k] = cadd(v[i + k], t);
-      }
-    }
-  }
+  stages<R, SIGN, 2>(v);
 }
 
 // nf arrays of pitch Fft2::P at buf, transformed in place; all threads of the block take part.

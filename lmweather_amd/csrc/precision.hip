@@ -1,7 +1,12 @@
-// Mixed-precision Legendre transforms on MFMA for gfx950: FP32 operands (v_mfma_f32_16x16x4_f32) or split BF16
-// hi+lo operands with three products hi*hi + hi*lo + lo*hi (v_mfma_f32_16x16x32_bf16), FP32 accumulation.  The
-// Legendre tables are built in FP64 (k_tables) and rounded/split once; the state, the FFTs, the grid products and
-// the RK4 update stay FP64.
+// Mixed-precision Legendre transforms on MFMA for gfx950, FP32 accumulation throughout:
+//   LMW_FP32    FP32 operands (v_mfma_f32_16x16x4_f32);
+//   LMW_BF16X3  operands split in two BF16 parts x = x1 + x2 (16 significant bits), three products
+//               x1 y1 + x1 y2 + x2 y1 (v_mfma_f32_16x16x32_bf16): relative error ~1e-5 per product;
+//   LMW_BF16X6  operands split in three BF16 parts (24 bits, an FP32 value), six products x1 y1 + x1 y2 + x2 y1 +
+//               x1 y3 + x2 y2 + x3 y1 (the dropped terms are below 2^-24 relative): FP32-level error at twice the
+//               MFMA work of BF16X3, which the memory-bound kernels mostly hide.
+// The Legendre tables are built in FP64 (k_tables) and rounded/split once; the state, the FFTs, the grid products
+// and the RK4 update stay FP64.
 //
 // Both transforms are, per zonal wavenumber m and parity of n - m, small GEMMs whose columns are ensemble members:
 //   inverse  F[jh][b] (quantity q) = sum_k' A[jh][k'] C_q[k'][b]      A = P or H rows of parity p, C = coefficients
@@ -25,6 +30,14 @@ __device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
   hi = static_cast<__bf16>(x);
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
+// x = x1 + x2 + x3 to the last bit of an FP32 value (each remainder is exact in FP32)
+__device__ inline void split_bf16x3(float x, __bf16 &x1, __bf16 &x2, __bf16 &x3) {
+  x1 = static_cast<__bf16>(x);
+  const float r1 = x - static_cast<float>(x1);
+  x2 = static_cast<__bf16>(r1);
+  x3 = static_cast<__bf16>(r1 - static_cast<float>(x2));
+}
+constexpr int nsplit(int prec) { return prec == LMW_BF16X6 ? 3 : 2; }
 
 // acc += a * b for one 16x16 tile; PREC = LMW_FP32 consumes one float per lane, LMW_BF16X3 eight bf16 pairs
 __device__ inline v4f mma32(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
@@ -32,6 +45,16 @@ __device__ inline v4f mma3(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f c) {
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+  return c;
+}
+// the six products of three-part operands, smallest first
+__device__ inline v4f mma6(v8bf a1, v8bf a2, v8bf a3, v8bf b1, v8bf b2, v8bf b3, v4f c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
   return c;
 }
 __device__ inline v8bf load8(const __bf16 *p) { return *reinterpret_cast<const v8bf *>(p); }
@@ -42,7 +65,7 @@ __device__ inline v8bf load8(const __bf16 *p) { return *reinterpret_cast<const v
 // Forward-transform A operand: FP32 [m][k = jh (P) | Jhp + jh (H)][r] and split BF16 [m][r][k], from the FP64
 // analysis tables Pa/Ha [m][jh][r].
 __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa, const double *__restrict__ Ha, float *__restrict__ A32,
-                                    __bf16 *__restrict__ Abh, __bf16 *__restrict__ Abl, const int *__restrict__ mrow) {
+                                    __bf16 *__restrict__ Abh, __bf16 *__restrict__ Abl, __bf16 *__restrict__ Ab3, const int *__restrict__ mrow) {
   const size_t total = static_cast<size_t>(2 * G.Jhp) * R;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += static_cast<size_t>(gridDim.x) * blockDim.x) {
     // e enumerates (global row gr, k) with k fastest: gr = poff[m] + r
@@ -52,10 +75,11 @@ __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa,
     const double *T = k < G.Jhp ? Pa : Ha;
     const float v = static_cast<float>(T[static_cast<size_t>(G.poff[m]) * G.Jhp + static_cast<size_t>(jh) * kp + r]);
     A32[static_cast<size_t>(2 * G.Jhp) * G.poff[m] + static_cast<size_t>(k) * kp + r] = v;
-    __bf16 hi, lo;
-    split_bf16(v, hi, lo);
-    Abh[e] = hi;
-    Abl[e] = lo;
+    __bf16 x1, x2, x3;
+    split_bf16x3(v, x1, x2, x3);  // x1, x2 are also the two-part split
+    Abh[e] = x1;
+    Abl[e] = x2;
+    Ab3[e] = x3;
   }
 }
 
@@ -64,7 +88,7 @@ __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa,
 __global__ void k_build_synth_tables(Geo G, int R, const double *__restrict__ Ps, const double *__restrict__ Hs, float *__restrict__ Ps32,
                                      float *__restrict__ Hs32, const int *__restrict__ soff, const int *__restrict__ ke32,
                                      const int *__restrict__ ko32, __bf16 *__restrict__ Sph, __bf16 *__restrict__ Spl, __bf16 *__restrict__ Shh,
-                                     __bf16 *__restrict__ Shl) {
+                                     __bf16 *__restrict__ Shl, __bf16 *__restrict__ Sp3, __bf16 *__restrict__ Sh3) {
   const size_t total = static_cast<size_t>(R) * G.Jhp;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += static_cast<size_t>(gridDim.x) * blockDim.x) {
     Ps32[e] = static_cast<float>(Ps[e]);
@@ -83,13 +107,15 @@ __global__ void k_build_synth_tables(Geo G, int R, const double *__restrict__ Ps
           vp = static_cast<float>(Ps[src]);
           vh = static_cast<float>(Hs[src]);
         }
-        __bf16 hi, lo;
-        split_bf16(vp, hi, lo);
-        Sph[base + t] = hi;
-        Spl[base + t] = lo;
-        split_bf16(vh, hi, lo);
-        Shh[base + t] = hi;
-        Shl[base + t] = lo;
+        __bf16 x1, x2, x3;
+        split_bf16x3(vp, x1, x2, x3);
+        Sph[base + t] = x1;
+        Spl[base + t] = x2;
+        Sp3[base + t] = x3;
+        split_bf16x3(vh, x1, x2, x3);
+        Shh[base + t] = x1;
+        Shl[base + t] = x2;
+        Sh3[base + t] = x3;
       }
     }
   }
@@ -99,13 +125,13 @@ __global__ void k_build_synth_tables(Geo G, int R, const double *__restrict__ Ps
 
 struct SynthTabs {
   const float *Ps32, *Hs32;
-  const __bf16 *Sph, *Spl, *Shh, *Shl;
+  const __bf16 *Sph, *Spl, *Shh, *Shl, *Sp3, *Sh3;
   const int *soff, *ke32, *ko32;
 };
 
 // Block: one m, 16 members, one wave per 16 northern latitudes.  Coefficients are staged in LDS in chunks of 32
 // rows of one parity: quantities q = zeta re/im, phi re/im, psi re/im, chi re/im (H uses psi, chi = q 4..7).
-// LDS per chunk: FP32 [q][k'][member] floats; BF16 [q][member][k'] hi and lo.
+// LDS per chunk: FP32 [q][k'][member] floats; BF16 [q][member][k'] per split part.
 // Tile shape (autotuned per truncation group): RC rows of one parity per LDS chunk (a multiple of 32, the K of one
 // BF16 MFMA).
 template <int PREC, int RC>
@@ -114,7 +140,8 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
   constexpr int SP = RC + 8;  // padded row pitch (bf16) of the BF16 chunk: distinct LDS banks across members
   extern __shared__ float lds_f[];
   float *cf = lds_f;                                                       // FP32 chunk [q][k'][member]
-  __bf16 *ch = reinterpret_cast<__bf16 *>(lds_f), *cl = ch + 8 * MEM * SP;  // BF16 chunk [q][member][k'], hi and lo
+  __bf16 *ch = reinterpret_cast<__bf16 *>(lds_f), *cl = ch + 8 * MEM * SP;  // BF16 chunk [q][member][k'], parts 1 and 2
+  __bf16 *c3 = cl + 8 * MEM * SP;                                           // part 3 (BF16X6)
   const int m = blockIdx.x, mb = blockIdx.y * MEM;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int Km = G.N - m + 1, kpe = G.kpe[m], kpo = G.kpo[m];
@@ -128,7 +155,7 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
   }
   for (int p = 0; p < 2; p++) {
     const int kp = p ? kpo : kpe;             // rows of this parity (16-padded)
-    const int kp32 = PREC == LMW_BF16X3 ? (p ? T.ko32[m] : T.ke32[m]) : kp;
+    const int kp32 = PREC != LMW_FP32 ? (p ? T.ko32[m] : T.ke32[m]) : kp;
     for (int c0 = 0; c0 < kp32; c0 += RC) {
       __syncthreads();
       // stage 8 quantities x RC rows x 16 members: one thread per (member, pair of rows), members fastest, so
@@ -145,7 +172,7 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
             z = spec[G.at(b0 + bl, 0, q)];
             d = spec[G.at(b0 + bl, 1, q)];
             f = spec[G.at(b0 + bl, 2, q)];
-            if (mixed && n == 0) f = cx(0, 0);  // phi_00 P_0^0 is added in FP64 below
+            if (mixed && n == 0) f = cx(0, 0);  // phi_00 P_0^0 (the area mean) stays out of the mixed path
           }
           const double kn = n >= 1 ? -a * a / (n * (n + 1.0)) : 0.0;  // psi, chi = kn (zeta, delta): sphere.cpp:185-190
           v[h][0] = static_cast<float>(z.x);
@@ -163,12 +190,13 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
             cf[(q * RC + kk) * MEM + j] = v[0][q];
             cf[(q * RC + kk + 1) * MEM + j] = v[1][q];
           } else {
-            __bf16 h0, l0, h1, l1;
-            split_bf16(v[0][q], h0, l0);
-            split_bf16(v[1][q], h1, l1);
+            __bf16 h0, l0, t0, h1, l1, t1;
+            split_bf16x3(v[0][q], h0, l0, t0);
+            split_bf16x3(v[1][q], h1, l1, t1);
             typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
             *reinterpret_cast<v2bf *>(ch + (q * MEM + j) * SP + kk) = v2bf{h0, h1};
             *reinterpret_cast<v2bf *>(cl + (q * MEM + j) * SP + kk) = v2bf{l0, l1};
+            if (PREC == LMW_BF16X6) *reinterpret_cast<v2bf *>(c3 + (q * MEM + j) * SP + kk) = v2bf{t0, t1};
           }
         }
       }
@@ -195,12 +223,23 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
           const size_t base = static_cast<size_t>(T.soff[m]) + (p ? static_cast<size_t>(G.Jhp) * T.ke32[m] : 0) +
                               static_cast<size_t>(jh) * kp32 + c0 + 32 * s + 8 * (lane >> 4);
           const v8bf aPh = load8(T.Sph + base), aPl = load8(T.Spl + base), aHh = load8(T.Shh + base), aHl = load8(T.Shl + base);
+          if constexpr (PREC == LMW_BF16X6) {
+            const v8bf aP3 = load8(T.Sp3 + base), aH3 = load8(T.Sh3 + base);
 #pragma unroll
-          for (int q = 0; q < 8; q++) {
-            const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
-            const v8bf bh = load8(ch + bo), bl = load8(cl + bo);
-            accP[p][q] = mma3(aPh, aPl, bh, bl, accP[p][q]);
-            if (q >= 4) accH[p][q - 4] = mma3(aHh, aHl, bh, bl, accH[p][q - 4]);
+            for (int q = 0; q < 8; q++) {
+              const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
+              const v8bf bh = load8(ch + bo), bl = load8(cl + bo), b3 = load8(c3 + bo);
+              accP[p][q] = mma6(aPh, aPl, aP3, bh, bl, b3, accP[p][q]);
+              if (q >= 4) accH[p][q - 4] = mma6(aHh, aHl, aH3, bh, bl, b3, accH[p][q - 4]);
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+              const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
+              const v8bf bh = load8(ch + bo), bl = load8(cl + bo);
+              accP[p][q] = mma3(aPh, aPl, bh, bl, accP[p][q]);
+              if (q >= 4) accH[p][q - 4] = mma3(aHh, aHl, bh, bl, accH[p][q - 4]);
+            }
           }
         }
       }
@@ -211,8 +250,6 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
   const int j = lane & 15, bl = mb + j;
   if (bl >= nb) return;
   const int b = b0 + bl;
-  double phi00 = 0;
-  if (mixed && m == 0) phi00 = spec[G.at(b, 2, 0)].x * 0.70710678118654752440;
   const double dm = m;
   for (int r = 0; r < 4; r++) {
     const int jh = jt * 16 + 4 * (lane >> 4) + r;
@@ -222,7 +259,9 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
       const int jj = s == 0 ? G.Jh + jh : G.Jh - 1 - jh;
       auto P = [&](int q) { return static_cast<double>(accP[0][q][r]) + sg * static_cast<double>(accP[1][q][r]); };
       auto H = [&](int q) { return sg * static_cast<double>(accH[0][q][r]) + static_cast<double>(accH[1][q][r]); };
-      const double2 Z = cx(P(0), P(1)), F = cx(P(2) + phi00, P(3)), psP = cx(P(4), P(5)), chP = cx(P(6), P(7));
+      // F = phi' = phi - phibar: the mean phi_00 P_0^0 stays out of the FP32 Fourier coefficients (the grid
+      // kernel needs phi' only, and the forward transform's epilogue adds the mean's flux -phibar delta in FP64)
+      const double2 Z = cx(P(0), P(1)), F = cx(P(2), P(3)), psP = cx(P(4), P(5)), chP = cx(P(6), P(7));
       const double2 psH = cx(H(0), H(1)), chH = cx(H(2), H(3));
       const double2 u = csub(cim(dm, chP), psH), v = cadd(cim(dm, psP), chH);  // sphere.cpp:200-201
       float2 *o = Fs + ((static_cast<size_t>(b) * G.J + jj) * G.M1 + m) * 4;
@@ -238,7 +277,7 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
 
 struct AnalTabs {
   const float *A32;
-  const __bf16 *Abh, *Abl;
+  const __bf16 *Abh, *Abl, *Ab3;
 };
 
 // Block: one (m, parity), 16 members, 4 waves over the 16-row tiles of that parity.  The K dimension runs over the
@@ -254,7 +293,8 @@ __global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2
   constexpr int TPW = (8 + NW - 1) / NW;
   extern __shared__ float lds_f[];
   float *bf = lds_f;                                                         // FP32 chunk [q][k][member]
-  __bf16 *bh = reinterpret_cast<__bf16 *>(lds_f), *blo = bh + 8 * MEM * AP;  // BF16 chunk [q][member][k], hi and lo
+  __bf16 *bh = reinterpret_cast<__bf16 *>(lds_f), *blo = bh + 8 * MEM * AP;  // BF16 chunk [q][member][k], parts 1 and 2
+  __bf16 *b3p = blo + 8 * MEM * AP;                                           // part 3 (BF16X6)
   const int m = blockIdx.x >> 1, p = blockIdx.x & 1, mb = blockIdx.y * MEM;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Rp = p ? G.kpo[m] : kpe, rbase = p ? kpe : 0;
@@ -290,12 +330,13 @@ __global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2
           bf[(q * KC + kk) * MEM + j] = vv[0][q];
           bf[(q * KC + kk + 1) * MEM + j] = vv[1][q];
         } else {
-          __bf16 h0, l0, h1, l1;
-          split_bf16(vv[0][q], h0, l0);
-          split_bf16(vv[1][q], h1, l1);
+          __bf16 h0, l0, t0, h1, l1, t1;
+          split_bf16x3(vv[0][q], h0, l0, t0);
+          split_bf16x3(vv[1][q], h1, l1, t1);
           typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
           *reinterpret_cast<v2bf *>(bh + (q * MEM + j) * AP + kk) = v2bf{h0, h1};
           *reinterpret_cast<v2bf *>(blo + (q * MEM + j) * AP + kk) = v2bf{l0, l1};
+          if (PREC == LMW_BF16X6) *reinterpret_cast<v2bf *>(b3p + (q * MEM + j) * AP + kk) = v2bf{t0, t1};
         }
       }
     }
@@ -317,10 +358,19 @@ __global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2
         for (int s = 0; s < kc / 32; s++) {
           const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + r) + c0 + 32 * s + 8 * (lane >> 4);
           const v8bf ah = load8(T.Abh + ai), al = load8(T.Abl + ai);
+          if constexpr (PREC == LMW_BF16X6) {
+            const v8bf a3 = load8(T.Ab3 + ai);
 #pragma unroll
-          for (int q = 0; q < 8; q++) {
-            const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
-            accv[ti][q] = mma3(ah, al, load8(bh + bo), load8(blo + bo), accv[ti][q]);
+            for (int q = 0; q < 8; q++) {
+              const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
+              accv[ti][q] = mma6(ah, al, a3, load8(bh + bo), load8(blo + bo), load8(b3p + bo), accv[ti][q]);
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+              const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
+              accv[ti][q] = mma3(ah, al, load8(bh + bo), load8(blo + bo), accv[ti][q]);
+            }
           }
         }
       }
@@ -379,10 +429,13 @@ void SphereCore::build_mixed_tables() {
   Spl.alloc(tot);
   Shh.alloc(tot);
   Shl.alloc(tot);
-  hipLaunchKernelGGL(k_build_anal_tables, dim3(1024), dim3(256), 0, stream, G, R, Pa.p, Ha.p, A32.p, Abh.p, Abl.p, d_mrow.p);
+  Sp3.alloc(tot);
+  Sh3.alloc(tot);
+  Ab3.alloc(na);
+  hipLaunchKernelGGL(k_build_anal_tables, dim3(1024), dim3(256), 0, stream, G, R, Pa.p, Ha.p, A32.p, Abh.p, Abl.p, Ab3.p, d_mrow.p);
   LMW_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_build_synth_tables, dim3(M1), dim3(256), 0, stream, G, R, Ps.p, Hs.p, Ps32.p, Hs32.p, d_soff.p, d_ke32.p, d_ko32.p,
-                     Sph.p, Spl.p, Shh.p, Shl.p);
+                     Sph.p, Spl.p, Shh.p, Shl.p, Sp3.p, Sh3.p);
   LMW_LAUNCH_CHECK();
   sync();
   mixed_ready = true;
@@ -395,10 +448,10 @@ const int kAnalNW[kAnalTiles] = {4, 4, 4, 8, 8, 8};
 
 template <int PREC, int RC>
 static void synth_mx_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
-  SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
+  SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Sp3.p, c.Sh3.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
   const int waves = (c.Jh + 15) / 16;
   dim3 grd(c.M1, (nb + MEM - 1) / MEM), blk(64 * waves);
-  const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : 2 * 8 * (RC + 8) * MEM * sizeof(__bf16);
+  const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : nsplit(PREC) * 8 * (RC + 8) * MEM * sizeof(__bf16);
   hipLaunchKernelGGL((k_synth_mx<PREC, RC>), grd, blk, lds, c.stream, c.geo(), T, spec, c.Fs32.p, b0, nb, 1);
   LMW_LAUNCH_CHECK();
 }
@@ -408,6 +461,9 @@ void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
   if (precision == LMW_FP32) {
     if (rc == 64) synth_mx_launch<LMW_FP32, 64>(*this, spec, b0, nb);
     else synth_mx_launch<LMW_FP32, 32>(*this, spec, b0, nb);
+  } else if (precision == LMW_BF16X6) {
+    if (rc == 64) synth_mx_launch<LMW_BF16X6, 64>(*this, spec, b0, nb);
+    else synth_mx_launch<LMW_BF16X6, 32>(*this, spec, b0, nb);
   } else {
     if (rc == 64) synth_mx_launch<LMW_BF16X3, 64>(*this, spec, b0, nb);
     else synth_mx_launch<LMW_BF16X3, 32>(*this, spec, b0, nb);
@@ -416,9 +472,9 @@ void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
 
 template <int PREC, int KC, int NW>
 static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
-  AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p};
+  AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p};
   dim3 grd(2 * c.M1, (nb + MEM - 1) / MEM), blk(64 * NW);
-  const size_t lds = PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float) : 2 * 8 * (KC + 8) * MEM * sizeof(__bf16);
+  const size_t lds = PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float) : nsplit(PREC) * 8 * (KC + 8) * MEM * sizeof(__bf16);
   hipLaunchKernelGGL((k_anal_mx<PREC, KC, NW>), grd, blk, lds, c.stream, c.geo(), T, c.Gb32.p, mode, stage, dt, c.prm.diffusion, c.x0.p, c.xs.p,
                      c.acc.p, raw_out, b0, nb, 1);
   LMW_LAUNCH_CHECK();
@@ -438,6 +494,7 @@ static void anal_mx_dispatch(SphereCore &c, int mode, int stage, double dt, int 
 
 void SphereCore::launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
   if (precision == LMW_FP32) anal_mx_dispatch<LMW_FP32>(*this, mode, stage, dt, b0, nb, raw_out);
+  else if (precision == LMW_BF16X6) anal_mx_dispatch<LMW_BF16X6>(*this, mode, stage, dt, b0, nb, raw_out);
   else anal_mx_dispatch<LMW_BF16X3>(*this, mode, stage, dt, b0, nb, raw_out);
 }
 

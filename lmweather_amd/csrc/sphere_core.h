@@ -65,7 +65,7 @@ struct SphereCore {
   bool mixed_ready = false;
   DevArray<int> d_mrow, d_soff, d_ke32, d_ko32;
   DevArray<float> A32, Ps32, Hs32;
-  DevArray<__bf16> Abh, Abl, Sph, Spl, Shh, Shl;
+  DevArray<__bf16> Abh, Abl, Sph, Spl, Shh, Shl, Ab3, Sp3, Sh3;  // BF16 parts 1, 2 (and 3 for BF16X6)
   DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K
   DevArray<double2> x0, xs, acc;               // [b][3][S]
   DevArray<double2> Fs;                        // [b][J][M1][4]

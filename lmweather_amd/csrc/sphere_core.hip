@@ -248,9 +248,11 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
   double2 *buf = lds, *tw = lds + 10 * P;
-  // mixed precision: the phi fluxes and the Bernoulli function are formed with phi - phibar (the area mean
-  // phi_00 / sqrt 2), whose exact contribution -phibar delta the forward transform's epilogue adds back
-  const double phibar = mixed ? xs[G.at(b, 2, 0)].x * 0.70710678118654752440 : 0.0;
+  // mixed precision: the phi rows hold phi' = phi - phibar (phibar = phi_00 / sqrt 2, the area mean), so the phi
+  // fluxes and the Bernoulli function are formed with phi'; the forward transform's epilogue adds the mean's exact
+  // contribution -phibar delta (the mean's part of the Bernoulli function only reaches n = 0, whose tendency is 0)
+  (void)mixed;
+  (void)xs;
   for (int i = tid; i < K; i += blockDim.x) tw[i] = twg[i];
   __syncthreads();
   for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
@@ -277,7 +279,7 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
       double o[5][2];
 #pragma unroll
       for (int e = 0; e < 2; e++) {
-        const double u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = (e ? Ph.y : Ph.x) - phibar;
+        const double u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = e ? Ph.y : Ph.x;
         // untilted f = 2 Omega (-cos(lon) cos(lat) 0 + mu 1) = (2 Omega) mu exactly (sphere.cpp:116)
         const double eta = z + (untilted ? (2 * G.omega) * mu[j] : fgrid[gi + e]);
         o[0][e] = u * eta;
@@ -920,7 +922,8 @@ void SphereCore::launch_to_grid(int mode, double *out, int b0, int nb) {
 }
 
 void SphereCore::set_precision(int p) {
-  if (p != LMW_FP64 && p != LMW_FP32 && p != LMW_BF16X3) invalid("precision must be LMW_FP64, LMW_FP32 or LMW_BF16X3");
+  if (p != LMW_FP64 && p != LMW_FP32 && p != LMW_BF16X3 && p != LMW_BF16X6)
+    invalid("precision must be LMW_FP64, LMW_FP32, LMW_BF16X3 or LMW_BF16X6");
   if (p != LMW_FP64) build_mixed_tables();
   const bool minor = p != LMW_FP64 && B > 1;
   if (minor != member_minor) {  // re-lay the state for the new precision's Legendre kernels
