@@ -103,6 +103,28 @@ __host__ __device__ inline double2 to_d2(float2 a) { return make_double2(a.x, a.
 template <class T> __host__ __device__ inline T from_d2(double2 a);
 template <> __host__ __device__ inline double2 from_d2<double2>(double2 a) { return a; }
 template <> __host__ __device__ inline float2 from_d2<float2>(double2 a) { return make_float2(static_cast<float>(a.x), static_cast<float>(a.y)); }
+// the same operations on float2 (FP32 grid-point work of the mixed-precision step)
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ inline float2 cim(float m, float2 a) { return make_float2(-m * a.y, m * a.x); }
+// complex number of type C from two doubles; the real type of a complex type
+template <class C> __host__ __device__ inline C mkc(double re, double im);
+template <> __host__ __device__ inline double2 mkc<double2>(double re, double im) { return make_double2(re, im); }
+template <> __host__ __device__ inline float2 mkc<float2>(double re, double im) { return make_float2(static_cast<float>(re), static_cast<float>(im)); }
+template <class C> struct real_of;
+template <> struct real_of<double2> { using type = double; };
+template <> struct real_of<float2> { using type = float; };
+// storage conversions between the complex types
+template <class C> __host__ __device__ inline C to_c(double2 a);
+template <class C> __host__ __device__ inline C to_c(float2 a);
+template <> __host__ __device__ inline double2 to_c<double2>(double2 a) { return a; }
+template <> __host__ __device__ inline double2 to_c<double2>(float2 a) { return make_double2(a.x, a.y); }
+template <> __host__ __device__ inline float2 to_c<float2>(double2 a) { return make_float2(static_cast<float>(a.x), static_cast<float>(a.y)); }
+template <> __host__ __device__ inline float2 to_c<float2>(float2 a) { return a; }
+
 // acc += a * s (complex times real)
 __device__ inline void cfma(double2 &acc, double2 a, double s) {
   acc.x = fma(a.x, s, acc.x);

@@ -45,40 +45,42 @@ __device__ inline double2 w16(int k) {
 }  // namespace fftd
 
 // v * W_16^K (SIGN < 0) or v * conj(W_16^K) (SIGN > 0), with the multiplications by 1, -i, +i and (1 -+ i)/sqrt 2
-// done as swaps, negations and one scaled sum/difference instead of a full complex product
-template <int K, int SIGN> __device__ inline double2 mul_w16(double2 v) {
-  constexpr double h = 0.7071067811865476;
+// done as swaps, negations and one scaled sum/difference instead of a full complex product.  C = double2 or float2.
+template <int K, int SIGN, class C> __device__ inline C mul_w16(C v) {
+  using T = typename real_of<C>::type;
+  constexpr T h = static_cast<T>(0.7071067811865476);
   constexpr int k = K & 7;
   if constexpr (k == 0) {
     return v;
   } else if constexpr (k == 4) {
-    return SIGN < 0 ? make_double2(v.y, -v.x) : make_double2(-v.y, v.x);  // * -i, * +i
+    return SIGN < 0 ? C{v.y, -v.x} : C{-v.y, v.x};  // * -i, * +i
   } else if constexpr (k == 2) {
-    return SIGN < 0 ? make_double2(h * (v.x + v.y), h * (v.y - v.x)) : make_double2(h * (v.x - v.y), h * (v.x + v.y));
+    return SIGN < 0 ? C{h * (v.x + v.y), h * (v.y - v.x)} : C{h * (v.x - v.y), h * (v.x + v.y)};
   } else if constexpr (k == 6) {
-    return SIGN < 0 ? make_double2(h * (v.y - v.x), -h * (v.x + v.y)) : make_double2(-h * (v.x + v.y), h * (v.x - v.y));
+    return SIGN < 0 ? C{h * (v.y - v.x), -h * (v.x + v.y)} : C{-h * (v.x + v.y), h * (v.x - v.y)};
   } else {
-    double2 w = fftd::w16(k);
+    const double2 wd = fftd::w16(k);
+    C w = mkc<C>(wd.x, wd.y);
     if (SIGN > 0) w.y = -w.y;
     return cmul(v, w);
   }
 }
 
-template <int R, int SIGN, int LEN, int I, int KK> __device__ inline void butterflies(double2 (&v)[R]) {
+template <int R, int SIGN, int LEN, int I, int KK, class C> __device__ inline void butterflies(C (&v)[R]) {
   if constexpr (KK < LEN / 2) {
-    const double2 t = mul_w16<KK * (16 / LEN), SIGN>(v[I + KK + LEN / 2]);
+    const C t = mul_w16<KK * (16 / LEN), SIGN>(v[I + KK + LEN / 2]);
     v[I + KK + LEN / 2] = csub(v[I + KK], t);
     v[I + KK] = cadd(v[I + KK], t);
     butterflies<R, SIGN, LEN, I, KK + 1>(v);
   }
 }
-template <int R, int SIGN, int LEN, int I> __device__ inline void stage(double2 (&v)[R]) {
+template <int R, int SIGN, int LEN, int I, class C> __device__ inline void stage(C (&v)[R]) {
   if constexpr (I < R) {
     butterflies<R, SIGN, LEN, I, 0>(v);
     stage<R, SIGN, LEN, I + LEN>(v);
   }
 }
-template <int R, int SIGN, int LEN> __device__ inline void stages(double2 (&v)[R]) {
+template <int R, int SIGN, int LEN, class C> __device__ inline void stages(C (&v)[R]) {
   if constexpr (LEN <= R) {
     stage<R, SIGN, LEN, 0>(v);
     stages<R, SIGN, 2 * LEN>(v);
@@ -86,12 +88,12 @@ template <int R, int SIGN, int LEN> __device__ inline void stages(double2 (&v)[R
 }
 
 // In-register DFT of length R (<= 16): v_k <- sum_n v_n exp(SIGN 2 pi i n k / R), radix-2 decimation in time.
-template <int R, int SIGN> __device__ inline void dft_reg(double2 (&v)[R]) {
+template <int R, int SIGN, class C> __device__ inline void dft_reg(C (&v)[R]) {
 #pragma unroll
   for (int i = 0; i < R; i++) {
     const int j = fftd::brev_c(i, R);
     if (i < j) {
-      const double2 t = v[i];
+      const C t = v[i];
       v[i] = v[j];
       v[j] = t;
     }
@@ -100,18 +102,18 @@ template <int R, int SIGN> __device__ inline void dft_reg(double2 (&v)[R]) {
 }
 
 // nf arrays of pitch Fft2::P at buf, transformed in place; all threads of the block take part.
-template <int R1, int R2, int SIGN> __device__ inline void fft2(double2 *buf, int nf, const double2 *tw) {
+template <int R1, int R2, int SIGN, class C> __device__ inline void fft2(C *buf, int nf, const C *tw) {
   using F = Fft2<R1, R2>;
   for (int it = threadIdx.x; it < nf * R2; it += blockDim.x) {
     const int f = it / R2, n2 = it - f * R2;
-    double2 *a = buf + f * F::P;
-    double2 v[R1];
+    C *a = buf + f * F::P;
+    C v[R1];
 #pragma unroll
     for (int n1 = 0; n1 < R1; n1++) v[n1] = a[F::phys(R2 * n1 + n2)];
     dft_reg<R1, SIGN>(v);
 #pragma unroll
     for (int k1 = 0; k1 < R1; k1++) {
-      double2 w = tw[2 * n2 * k1];
+      C w = tw[2 * n2 * k1];
       if (SIGN > 0) w.y = -w.y;
       a[F::phys(R2 * k1 + n2)] = cmul(v[k1], w);
     }
@@ -119,8 +121,8 @@ template <int R1, int R2, int SIGN> __device__ inline void fft2(double2 *buf, in
   __syncthreads();
   for (int it = threadIdx.x; it < nf * R1; it += blockDim.x) {
     const int f = it / R1, k1 = it - f * R1;
-    double2 *a = buf + f * F::P;
-    double2 v[R2];
+    C *a = buf + f * F::P;
+    C v[R2];
 #pragma unroll
     for (int n2 = 0; n2 < R2; n2++) v[n2] = a[F::phys(R2 * k1 + n2)];
     dft_reg<R2, SIGN>(v);

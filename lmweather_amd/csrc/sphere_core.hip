@@ -58,40 +58,41 @@ __device__ void fft_lds(double2 *buf, int nf, int L, int logL, const double2 *tw
 // reference's separate transform of each row does, sphere.cpp:123-136).  tw[q] = exp(-2 pi i q / K), q < L.
 //
 // Forward: arr = FFT_L(x[2k] + i x[2k+1]); returns X_m / K for 0 <= m < L (fourier_forward: a[m] / nlon).
-template <class F> __device__ inline double2 rfft_coef(const double2 *arr, int m, const double2 *tw, double invK) {
-  const double2 Zm = arr[F::phys(F::pos(m))], Zc = arr[F::phys(F::pos((F::L - m) & (F::L - 1)))];
-  const double2 Fe = cx(0.5 * (Zm.x + Zc.x), 0.5 * (Zm.y - Zc.y));  // (Zm + conj Zc) / 2: even samples
-  const double2 Fo = cx(0.5 * (Zm.y + Zc.y), 0.5 * (Zc.x - Zm.x));  // (Zm - conj Zc) / 2i: odd samples
+template <class F, class C> __device__ inline C rfft_coef(const C *arr, int m, const C *tw, typename real_of<C>::type invK) {
+  using T = typename real_of<C>::type;
+  const T half = static_cast<T>(0.5);
+  const C Zm = arr[F::phys(F::pos(m))], Zc = arr[F::phys(F::pos((F::L - m) & (F::L - 1)))];
+  const C Fe = C{half * (Zm.x + Zc.x), half * (Zm.y - Zc.y)};  // (Zm + conj Zc) / 2: even samples
+  const C Fo = C{half * (Zm.y + Zc.y), half * (Zc.x - Zm.x)};  // (Zm - conj Zc) / 2i: odd samples
   return cscale(cadd(Fe, cmul(tw[m], Fo)), invK);
 }
 
 // Backward: entry m' (0 <= m' < L) of the half-length spectrum whose backward transform is x[2k] + i x[2k+1] for
 // the real row x_k = sum_m a_m exp(2 pi i m k / K) with a_0 = Re F_0, a_m = F_m, a_{K-m} = conj F_m (m = 1..N) and
 // zero elsewhere (fourier_backward, sphere.cpp:130-136).  F_m is read at src[4 m].
-template <class TS> __device__ inline double2 irfft_entry(const TS *__restrict__ src, int mp, int L, int N, const double2 *tw) {
-  double2 lo = cx(0, 0), hi = cx(0, 0);  // a_{m'} and a_{m'+L}
-  if (mp == 0) lo = cx(to_d2(src[0]).x, 0.0);
-  else if (mp <= N) lo = to_d2(src[4 * mp]);
-  if (mp >= 1 && L - mp <= N) hi = cconj(to_d2(src[4 * (L - mp)]));
-  const double2 E = cadd(lo, hi), O = cmul(csub(lo, hi), cconj(tw[mp]));
-  return cx(E.x - O.y, E.y + O.x);  // E + i O
+template <class TS, class C> __device__ inline C irfft_entry(const TS *__restrict__ src, int mp, int L, int N, const C *tw) {
+  C lo = mkc<C>(0, 0), hi = lo;  // a_{m'} and a_{m'+L}
+  if (mp == 0) lo = C{to_c<C>(src[0]).x, 0};
+  else if (mp <= N) lo = to_c<C>(src[4 * mp]);
+  if (mp >= 1 && L - mp <= N) hi = cconj(to_c<C>(src[4 * (L - mp)]));
+  const C E = cadd(lo, hi), O = cmul(csub(lo, hi), cconj(tw[mp]));
+  return C{E.x - O.y, E.y + O.x};  // E + i O
 }
 
 // Writes the analysis operands of one (member, m, latitude pair) from the per-latitude values (north n, south s).
 // Layout per (b, m): [parity p][part: 0 = P operand X, 1 = H operand Y][jh][4], where even rows (p = 0) use the
 // north/south sums X+ = Xn + Xs and differences Y- = Yn - Ys, odd rows X- and Y+ (P(-mu) = (-1)^(n-m) P(mu),
 // H(-mu) = -(-1)^(n-m) H(mu)).
-template <class TS>
-__device__ inline void store_operands(TS *Gb, size_t bm, int Jh, int jh, const double2 Xn[4], const double2 Xs[4], const double2 Yn[4],
-                                      const double2 Ys[4]) {
+template <class TS, class C>
+__device__ inline void store_operands(TS *Gb, size_t bm, int Jh, int jh, const C Xn[4], const C Xs[4], const C Yn[4], const C Ys[4]) {
   TS *o = Gb + bm * 16 * Jh + static_cast<size_t>(jh) * 4;
   const size_t part = static_cast<size_t>(Jh) * 4;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    o[i] = from_d2<TS>(cadd(Xn[i], Xs[i]));             // p 0, X
-    o[part + i] = from_d2<TS>(csub(Yn[i], Ys[i]));      // p 0, Y
-    o[2 * part + i] = from_d2<TS>(csub(Xn[i], Xs[i]));  // p 1, X
-    o[3 * part + i] = from_d2<TS>(cadd(Yn[i], Ys[i]));  // p 1, Y
+    o[i] = to_c<TS>(cadd(Xn[i], Xs[i]));             // p 0, X
+    o[part + i] = to_c<TS>(csub(Yn[i], Ys[i]));      // p 0, Y
+    o[2 * part + i] = to_c<TS>(csub(Xn[i], Xs[i]));  // p 1, X
+    o[3 * part + i] = to_c<TS>(cadd(Yn[i], Ys[i]));  // p 1, Y
   }
 }
 }  // namespace
@@ -235,25 +236,29 @@ __global__ __launch_bounds__(64 * MB) void k_synth(Geo G, const double *__restri
 // Rows in LDS: inverse U, V, zeta, phi and forward A, B, U phi, V phi, phi + g hs + E, each at (north, south).
 // Storage TS of the Fourier coefficients and analysis operands: double2 (FP64 path) or float2 (mixed precision).
 // mflags[b]: bit 0 = rotation axis untilted (f = 2 Omega mu computed, not read), bit 1 = no orography (hs = 0).
+// Grid-point part of one RK4 stage for one latitude pair and member: inverse FFTs of U, V, zeta, phi (north and
+// south rows), the nonlinear products (sphere.cpp:348-358), forward FFTs of the five products and the analysis
+// operands.  TS is the storage type of the Fourier rows and operands; the arithmetic runs in its precision
+// (double2: the FP64 step; float2: the mixed-precision step, whose rows hold FP32 values already).
 template <int R1, int R2, class TS>
 __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__ Fs, const double *__restrict__ fgrid,
                                                    const double *__restrict__ hs, const double2 *__restrict__ twg,
                                                    const double *__restrict__ mu, const double *__restrict__ w,
-                                                   TS *__restrict__ Gb, int b0, int mixed, const double2 *__restrict__ xs,
-                                                   const int *__restrict__ mflags) {
+                                                   TS *__restrict__ Gb, int b0, const int *__restrict__ mflags) {
   using F = Fft2<R1, R2>;
+  using C = TS;
+  using T = typename real_of<C>::type;
   constexpr int L = F::L, P = F::P;
-  extern __shared__ double2 lds[];
+  extern __shared__ double2 lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
   const int K = 2 * L, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
-  double2 *buf = lds, *tw = lds + 10 * P;
+  C *buf = lds, *tw = lds + 10 * P;
   // mixed precision: the phi rows hold phi' = phi - phibar (phibar = phi_00 / sqrt 2, the area mean), so the phi
   // fluxes and the Bernoulli function are formed with phi'; the forward transform's epilogue adds the mean's exact
   // contribution -phibar delta (the mean's part of the Bernoulli function only reaches n = 0, whose tendency is 0)
-  (void)mixed;
-  (void)xs;
-  for (int i = tid; i < K; i += blockDim.x) tw[i] = twg[i];
+  for (int i = tid; i < K; i += blockDim.x) tw[i] = to_c<C>(twg[i]);
   __syncthreads();
   for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
     const int r = idx / L, mp = idx - r * L;
@@ -263,33 +268,34 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   }
   __syncthreads();
   fft2<R1, R2, +1>(buf, 8, tw);
-  const double om = 1 - mu[jn] * mu[jn];  // identical for the southern row: mu(s) = -mu(n)
-  const double g = G.g;
+  const T om = static_cast<T>(1 - mu[jn] * mu[jn]);  // identical for the southern row: mu(s) = -mu(n)
+  const T g = static_cast<T>(G.g);
   const int flags = mflags[b];
   const bool untilted = flags & 1, flat = flags & 2;
-  double2 res[10];
+  C res[10];
   const int k = tid;  // one pair of longitudes (2k, 2k+1) per thread: L <= 256
   if (k < L) {
     const int pk = F::phys(F::pos(k));
 #pragma unroll
     for (int s = 0; s < 2; s++) {
       const int j = s ? js : jn;
-      const double2 U = buf[(0 + s) * P + pk], V = buf[(2 + s) * P + pk], Z = buf[(4 + s) * P + pk], Ph = buf[(6 + s) * P + pk];
+      const C U = buf[(0 + s) * P + pk], V = buf[(2 + s) * P + pk], Z = buf[(4 + s) * P + pk], Ph = buf[(6 + s) * P + pk];
       const size_t gi = (static_cast<size_t>(b) * G.J + j) * K + 2 * k;
-      double o[5][2];
+      const T fj = static_cast<T>((2 * G.omega) * mu[j]);
+      T o[5][2];
 #pragma unroll
       for (int e = 0; e < 2; e++) {
-        const double u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = e ? Ph.y : Ph.x;
+        const T u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = e ? Ph.y : Ph.x;
         // untilted f = 2 Omega (-cos(lon) cos(lat) 0 + mu 1) = (2 Omega) mu exactly (sphere.cpp:116)
-        const double eta = z + (untilted ? (2 * G.omega) * mu[j] : fgrid[gi + e]);
+        const T eta = z + (untilted ? fj : static_cast<T>(fgrid[gi + e]));
         o[0][e] = u * eta;
         o[1][e] = v * eta;
         o[2][e] = u * ph;
         o[3][e] = v * ph;
-        o[4][e] = ph + g * (flat ? 0.0 : hs[gi + e]) + (u * u + v * v) / (2 * om);  // Bernoulli: g (h + hs) + E (sphere.cpp:357)
+        o[4][e] = ph + g * (flat ? T(0) : static_cast<T>(hs[gi + e])) + (u * u + v * v) / (2 * om);  // g (h + hs) + E (sphere.cpp:357)
       }
 #pragma unroll
-      for (int q = 0; q < 5; q++) res[2 * q + s] = cx(o[q][0], o[q][1]);
+      for (int q = 0; q < 5; q++) res[2 * q + s] = C{o[q][0], o[q][1]};
     }
   }
   __syncthreads();
@@ -300,24 +306,24 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   }
   __syncthreads();
   fft2<R1, R2, -1>(buf, 10, tw);
-  const double a = G.a, wj = w[jn], wd = w[jn] / (1 - mu[jn] * mu[jn]) / a, invK = 1.0 / K;
+  const T wj = static_cast<T>(w[jn]), wd = static_cast<T>(w[jn] / (1 - mu[jn] * mu[jn]) / G.a), invK = static_cast<T>(1.0 / K);
   for (int m = tid; m < M1; m += blockDim.x) {
-    double2 c[10];
+    C c[10];
 #pragma unroll
     for (int r = 0; r < 10; r++) c[r] = rfft_coef<F>(buf + r * P, m, tw, invK);
-    const double dm = m;
+    const T dm = static_cast<T>(m);
     // dzeta -= (i m A wd) P - (B wd) H; ddelta += (i m B wd) P + (A wd) H + (E w) P n(n+1)/a^2;
     // dphi -= (i m U phi wd) P - (V phi wd) H   (sphere.cpp:374-383); index 0 north, 1 south
-    double2 X[2][4], Y[2][4];
+    C X[2][4], Y[2][4];
 #pragma unroll
     for (int s = 0; s < 2; s++) {
-      const double2 A = c[0 + s], B = c[2 + s], UP = c[4 + s], VP = c[6 + s], PE = c[8 + s];
+      const C A = c[0 + s], B = c[2 + s], UP = c[4 + s], VP = c[6 + s], PE = c[8 + s];
       X[s][0] = cscale(cim(dm, A), -wd);
       Y[s][0] = cscale(B, wd);
       X[s][1] = cscale(cim(dm, B), wd);
       Y[s][1] = cscale(A, wd);
       X[s][2] = cscale(PE, wj);
-      Y[s][2] = cx(0, 0);
+      Y[s][2] = mkc<C>(0, 0);
       X[s][3] = cscale(cim(dm, UP), -wd);
       Y[s][3] = cscale(VP, wd);
     }
@@ -884,12 +890,12 @@ static size_t grid_lds_bytes(int K) {
   }
 
 void SphereCore::launch_tend_grid(int b0, int nb, int mixed) {
-  if (mixed) {
-    LMW_DISPATCH_LT(K / 2, k_tend_grid, float2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs32.p, f.p, hs.p, tw.p, mu.p, w.p,
-                    Gb32.p, b0, mixed, xs.p, mflags.p)
+  if (mixed) {  // FP32 rows, FP32 arithmetic: half the LDS of the FP64 kernel
+    LMW_DISPATCH_LT(K / 2, k_tend_grid, float2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K) / 2, stream, geo(), Fs32.p, f.p, hs.p, tw.p, mu.p,
+                    w.p, Gb32.p, b0, mflags.p)
   } else {
     LMW_DISPATCH_LT(K / 2, k_tend_grid, double2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs.p, f.p, hs.p, tw.p, mu.p, w.p,
-                    Gb.p, b0, mixed, xs.p, mflags.p)
+                    Gb.p, b0, mflags.p)
   }
   LMW_LAUNCH_CHECK();
 }
