@@ -35,6 +35,19 @@ DTYPES = {"fp64": "f64", "fp32": "f32 Legendre, f64 state/FFT", "bf16x3": "bf16x
 HBM_PEAK = 8.0e12
 
 
+def measured_traffic(name: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this exact
+    workload (profiles/<name>_traffic.txt, copy-calibrated), or None when there is no summary for it."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1].split("::")[-1] == kernel:
+            return float(f[2])
+    return None
+
+
 def initial_state(lmw):
     from lmweather_amd.initial import gaussian_orography, random_balanced_state
     base = lmw.Sphere(N, lmw.SphereParams(A, OM, G))
@@ -164,7 +177,8 @@ def plane_bench(lmw, args) -> dict:
         "config": {"workload": f"beta-plane channel 1024x512 (dx 10 km), sech^2 jet + 1e-3 m noise, batch {args.members}, "
                                f"RK4 dt {dt:.2f} s", "members": args.members, "dt_s": dt},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": gbs * 1e9 / HBM_PEAK,
-                     "traffic": None, "kernel": "k_plane_stage", "work_per_launch": f"{bpc / 4 * cells * args.members / 1e6:.1f} MB"},
+                     "traffic": measured_traffic(f"plane_{nx}x{ny}x{args.members}_{args.precision}", "k_plane_stage"),
+                     "kernel": "k_plane_stage", "work_per_launch": f"{bpc / 4 * cells * args.members / 1e6:.1f} MB"},
         "mass_drift": (pl.mean_depth(0) - m0) / m0,
     }
 
@@ -264,7 +278,7 @@ def main() -> int:
     ap.add_argument("--members", type=int, default=32)
     ap.add_argument("--precision", default="bf16x6", choices=list(PREC_NAMES))
     ap.add_argument("--check-days", type=float, default=1.0)
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-batch", action="store_true", help="time the batch only (clean kernel profiles)")
     args = ap.parse_args()
@@ -307,14 +321,16 @@ def main() -> int:
                                  "tflops": anal_f / avg[2] / 1e12, "frac": mult * anal_f / avg[2] / peak},
     }
     dom = int(np.argmax(ms))
+    prof_name = f"sphere_T{N}x{args.members}_{args.precision}"
     if dom == 1:
         roof = {"bound": "hbm", "achieved": grid_b / avg[1] / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": grid_b / avg[1] / HBM_PEAK, "traffic": None, "kernel": "k_tend_grid",
-                "work_per_launch": f"{grid_b / 1e6:.1f} MB"}
+                "frac": grid_b / avg[1] / HBM_PEAK, "traffic": measured_traffic(prof_name, "k_tend_grid"),
+                "kernel": "k_tend_grid", "work_per_launch": f"{grid_b / 1e6:.1f} MB"}
     else:
         fl = synth_f if dom == 0 else anal_f
+        kname = kernels[list(kernels)[dom]]["kernel"]
         roof = {"bound": "mfma", "achieved": fl / avg[dom] / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": mult * fl / avg[dom] / peak, "traffic": None, "kernel": kernels[list(kernels)[dom]]["kernel"],
+                "frac": mult * fl / avg[dom] / peak, "traffic": measured_traffic(prof_name, kname), "kernel": kname,
                 "work_per_launch": f"{fl / 1e9:.3f} GFLOP"}
 
     out = {
