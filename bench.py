@@ -223,6 +223,8 @@ def ensemble_bench(lmw, args) -> dict:
     def run(precision):
         ens = build_uneven(lmw, precision, spec)
         tiles, tune_ms = ens.autotune()
+        if args.persistent and precision != lmw.FP64:
+            ens.set_persistent(True)
         ens.macro_steps(dts, subs, args.warmup)
         t0 = time.perf_counter()
         ens.macro_steps(dts, subs, args.steps)  # synchronous: returns when the device is done
@@ -254,6 +256,7 @@ def ensemble_bench(lmw, args) -> dict:
                                "one step = one 600 s macro step (HIP graph of 1/2/4 RK4 steps per group)",
                    "members": nmembers, "simulated_days": days, "precision": args.precision},
         "t170_member_days_per_s": 32 * days / wall,
+        "legendre_kernels": "persistent multi-group" if args.persistent else "per group, groups as graph branches",
         "finite": bool(fin),
         "tiles": {str(t): {"fp64_mb": lmw.sphere.tile_variant(0, int(tl[0]))[0],
                            "mixed_synth_rc": lmw.sphere.tile_variant(1, int(tl[2]))[0],
@@ -281,6 +284,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-batch", action="store_true", help="time the batch only (clean kernel profiles)")
+    ap.add_argument("--persistent", action="store_true", help="ensemble: persistent multi-group Legendre kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is a design, not a run",

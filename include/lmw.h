@@ -214,6 +214,11 @@ lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const in
  * groups' streams, so the truncations run concurrently and the host launches one graph per macro step (no reference
  * counterpart) */
 lmw_status lmw_ensemble_macro_steps(lmw_ensemble *e, const double *dts, const int *substeps, int64_t count);
+/* persistent macro steps (mixed-precision ensembles of at most 4 truncation groups whose sub-step counts divide the
+ * largest): in each stage, one persistent kernel walks a static work list of the inverse Legendre transforms of
+ * every group due a step, the groups' grid-point kernels follow, then one persistent kernel for the forward
+ * transforms; captured as one HIP graph per macro step like the default mode (no reference counterpart) */
+lmw_status lmw_ensemble_set_persistent(lmw_ensemble *e, int on);
 /* Tile shapes of the Legendre kernels, chosen per truncation group by timing every variant on the group's state
  * (restored afterwards) and cached in $LMW_TUNE_CACHE (default ~/.cache/lmweather_amd/tiles_v1.txt) keyed by
  * architecture, truncation, members and precision; force != 0 re-times.  tiles[4 g + 0..3] = variant indices of

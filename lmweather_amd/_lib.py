@@ -108,6 +108,7 @@ SIGNATURES = {
     "lmw_ensemble_steps_graph": [_vp, _dp, _i64p],
     "lmw_ensemble_steps_timed": [_vp, _dp, _i64p, _dp],
     "lmw_ensemble_macro_steps": [_vp, _dp, _ip, C.c_int64],
+    "lmw_ensemble_set_persistent": [_vp, C.c_int],
     "lmw_ensemble_autotune": [_vp, C.c_int, _ip, _dp],
     "lmw_sphere_autotune": [_vp, C.c_int, _ip, _dp],
     "lmw_tile_variants": [C.c_int, C.c_int, _ip, _ip],

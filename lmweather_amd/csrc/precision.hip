@@ -13,7 +13,9 @@
 //   forward  O_q[r][b]            = sum_jh P[jh][r] X_q[jh][b] + H[jh][r] Y_q[jh][b]
 // Each 16 x 16 MFMA tile holds one quantity for 16 members, so one lane's accumulators carry every quantity of
 // its member and the epilogues (north/south combination, RK4 update) need no data exchange.
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "sphere_core.h"
 #include "sphere_epilogue.h"
@@ -134,15 +136,15 @@ struct SynthTabs {
 // LDS per chunk: FP32 [q][k'][member] floats; BF16 [q][member][k'] per split part.
 // Tile shape (autotuned per truncation group): RC rows of one parity per LDS chunk (a multiple of 32, the K of one
 // BF16 MFMA).
+// The work of one block (order m, members mb .. mb + 15); shared by the per-group and the multi-group kernels.
 template <int PREC, int RC>
-__global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
-                                                  int nb, int mixed) {
+__device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T, const double2 *__restrict__ spec, float2 *__restrict__ Fs,
+                                               int b0, int nb, int mixed, int m, int mb) {
   constexpr int SP = RC + 8;  // padded row pitch (bf16) of the BF16 chunk: distinct LDS banks across members
   extern __shared__ float lds_f[];
   float *cf = lds_f;                                                       // FP32 chunk [q][k'][member]
   __bf16 *ch = reinterpret_cast<__bf16 *>(lds_f), *cl = ch + 8 * MEM * SP;  // BF16 chunk [q][member][k'], parts 1 and 2
   __bf16 *c3 = cl + 8 * MEM * SP;                                           // part 3 (BF16X6)
-  const int m = blockIdx.x, mb = blockIdx.y * MEM;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int Km = G.N - m + 1, kpe = G.kpe[m], kpo = G.kpo[m];
   const double a = G.a;
@@ -285,17 +287,17 @@ struct AnalTabs {
 // o4 re/im (dzeta, ddelta, the Laplacian part of ddelta, dphi).
 // Tile shape (autotuned per truncation group): KC latitudes of K per LDS chunk, NW waves per block, each wave
 // owning row tiles wave, wave + NW, ... (TPW of them: 8 tiles of 16 rows cover the largest parity block, T170).
+// The work of one block (order m, parity p, members mb .. mb + 15), shared by the per-group and multi-group kernels.
 template <int PREC, int KC, int NW>
-__global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
-                                                 double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
-                                                 double2 *__restrict__ raw, int b0, int nb, int mixed) {
+__device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, const float2 *__restrict__ Gb, int mode, int stage, double dt,
+                                              double nu4, double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
+                                              double2 *__restrict__ raw, int b0, int nb, int mixed, int m, int p, int mb) {
   constexpr int AP = KC + 8;         // padded row pitch (bf16) of the BF16 chunk
   constexpr int TPW = (8 + NW - 1) / NW;
   extern __shared__ float lds_f[];
   float *bf = lds_f;                                                         // FP32 chunk [q][k][member]
   __bf16 *bh = reinterpret_cast<__bf16 *>(lds_f), *blo = bh + 8 * MEM * AP;  // BF16 chunk [q][member][k], parts 1 and 2
   __bf16 *b3p = blo + 8 * MEM * AP;                                           // part 3 (BF16X6)
-  const int m = blockIdx.x >> 1, p = blockIdx.x & 1, mb = blockIdx.y * MEM;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Rp = p ? G.kpo[m] : kpe, rbase = p ? kpe : 0;
   if (Rp == 0) return;
@@ -391,6 +393,68 @@ __global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2
       const double2 o3 = cx(accv[ti][4][r], accv[ti][5][r]), o4 = cx(accv[ti][6][r], accv[ti][7][r]);
       anal_epilogue(G, mode, stage, dt, nu4, b, bl, m, k, o1, o2, o3, o4, x0, xs, acc, raw, mixed);
     }
+  }
+}
+
+template <int PREC, int RC>
+__global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
+                                                  int nb, int mixed) {
+  synth_mx_block<PREC, RC>(G, T, spec, Fs, b0, nb, mixed, blockIdx.x, blockIdx.y * MEM);
+}
+
+template <int PREC, int KC, int NW>
+__global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
+                                                 double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
+                                                 double2 *__restrict__ raw, int b0, int nb, int mixed) {
+  anal_mx_block<PREC, KC, NW>(G, T, Gb, mode, stage, dt, nu4, x0, xs, acc, raw, b0, nb, mixed, blockIdx.x >> 1, blockIdx.x & 1, blockIdx.y * MEM);
+}
+
+// ------------------------------------------------------------------------------------------- multi-group kernels
+// Persistent kernels over the truncation groups of an ensemble: a fixed grid of blocks walks a static work list
+// (block w takes items w, w + gridDim.x, ...; the list is sorted by decreasing cost), so one launch per stage phase
+// serves every active group and the small truncations fill the gaps of the large ones.  No counters: the schedule
+// is fixed when the list is built.
+
+struct SynthGroup {
+  Geo G;
+  SynthTabs T;
+  const double2 *spec;
+  float2 *Fs;
+  int nb;
+};
+struct MultiSynth {
+  SynthGroup g[kMaxGroups];
+  const int4 *work;  // (group, m, member tile start, 0)
+  int nwork;
+};
+template <int PREC, int RC>
+__global__ __launch_bounds__(512) void k_synth_mx_multi(MultiSynth M) {
+  for (int w = blockIdx.x; w < M.nwork; w += gridDim.x) {
+    const int4 it = M.work[w];
+    const SynthGroup &g = M.g[it.x];
+    synth_mx_block<PREC, RC>(g.G, g.T, g.spec, g.Fs, 0, g.nb, 1, it.y, it.z);
+  }
+}
+
+struct AnalGroup {
+  Geo G;
+  AnalTabs T;
+  const float2 *Gb;
+  double2 *x0, *xs, *acc;
+  double dt, nu4;
+  int nb;
+};
+struct MultiAnal {
+  AnalGroup g[kMaxGroups];
+  const int4 *work;  // (group, m, parity, member tile start)
+  int nwork, stage;
+};
+template <int PREC, int KC, int NW>
+__global__ __launch_bounds__(512) void k_anal_mx_multi(MultiAnal M) {
+  for (int w = blockIdx.x; w < M.nwork; w += gridDim.x) {
+    const int4 it = M.work[w];
+    const AnalGroup &g = M.g[it.x];
+    anal_mx_block<PREC, KC, NW>(g.G, g.T, g.Gb, ANAL_TEND, M.stage, g.dt, g.nu4, g.x0, g.xs, g.acc, nullptr, 0, g.nb, 1, it.y, it.z, it.w);
   }
 }
 
@@ -496,6 +560,121 @@ void SphereCore::launch_anal_mx(int mode, int stage, double dt, int b0, int nb, 
   if (precision == LMW_FP32) anal_mx_dispatch<LMW_FP32>(*this, mode, stage, dt, b0, nb, raw_out);
   else if (precision == LMW_BF16X6) anal_mx_dispatch<LMW_BF16X6>(*this, mode, stage, dt, b0, nb, raw_out);
   else anal_mx_dispatch<LMW_BF16X3>(*this, mode, stage, dt, b0, nb, raw_out);
+}
+
+// ------------------------------------------------------------------------------------------- multi-group host side
+
+void build_multi_work(const std::vector<SphereCore *> &groups, unsigned active_mask, MultiWork &w) {
+  struct Item {
+    int4 v;
+    double cost;
+  };
+  std::vector<Item> sy, an;
+  for (int g = 0; g < static_cast<int>(groups.size()); g++) {
+    if (!(active_mask & (1u << g))) continue;
+    const SphereCore &c = *groups[g];
+    for (int m = 0; m <= c.N; m++)
+      for (int mb = 0; mb < c.B; mb += MEM) {
+        const int Km = c.N - m + 1;
+        sy.push_back({make_int4(g, m, mb, 0), static_cast<double>(Km) * c.Jh});
+        for (int p = 0; p < 2; p++) {
+          const int rows = p ? c.h_kpo[m] : c.h_kpe[m];
+          if (rows > 0) an.push_back({make_int4(g, m, p, mb), static_cast<double>(rows) * 2 * c.Jhp});
+        }
+      }
+  }
+  auto by_cost = [](const Item &a, const Item &b) { return a.cost > b.cost; };
+  std::stable_sort(sy.begin(), sy.end(), by_cost);
+  std::stable_sort(an.begin(), an.end(), by_cost);
+  std::vector<int4> vs(sy.size()), va(an.size());
+  for (size_t i = 0; i < sy.size(); i++) vs[i] = sy[i].v;
+  for (size_t i = 0; i < an.size(); i++) va[i] = an[i].v;
+  w.synth.alloc(std::max<size_t>(vs.size(), 1));
+  w.anal.alloc(std::max<size_t>(va.size(), 1));
+  if (!vs.empty()) LMW_HIP(hipMemcpy(w.synth.p, vs.data(), vs.size() * sizeof(int4), hipMemcpyHostToDevice));
+  if (!va.empty()) LMW_HIP(hipMemcpy(w.anal.p, va.data(), va.size() * sizeof(int4), hipMemcpyHostToDevice));
+  w.nsynth = static_cast<int>(vs.size());
+  w.nanal = static_cast<int>(va.size());
+}
+
+// resident blocks of `kernel` on the whole device, at most `want`
+template <class K> static int persistent_blocks(K kernel, int threads, size_t lds, int want) {
+  int per_cu = 0, dev = 0;
+  LMW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+  LMW_HIP(hipGetDevice(&dev));
+  int cus = 0;
+  LMW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(1, std::min(want, std::max(1, per_cu) * cus));
+}
+
+template <int PREC, int RC>
+static void synth_multi(const std::vector<SphereCore *> &gs, const MultiWork &w, hipStream_t stream) {
+  MultiSynth M{};
+  int waves = 1;
+  for (size_t g = 0; g < gs.size(); g++) {
+    SphereCore &c = *gs[g];
+    M.g[g] = SynthGroup{c.geo(), SynthTabs{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Sp3.p, c.Sh3.p, c.d_soff.p, c.d_ke32.p,
+                                             c.d_ko32.p},
+                        c.xs.p, c.Fs32.p, c.B};
+    waves = std::max(waves, (c.Jh + 15) / 16);
+  }
+  M.work = w.synth.p;
+  M.nwork = w.nsynth;
+  const int threads = 64 * waves;
+  const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : nsplit(PREC) * 8 * (RC + 8) * MEM * sizeof(__bf16);
+  const int nblk = persistent_blocks(k_synth_mx_multi<PREC, RC>, threads, lds, w.nsynth);
+  hipLaunchKernelGGL((k_synth_mx_multi<PREC, RC>), dim3(nblk), dim3(threads), lds, stream, M);
+  LMW_LAUNCH_CHECK();
+}
+
+void launch_synth_mx_multi(const std::vector<SphereCore *> &gs, const MultiWork &w, hipStream_t stream) {
+  if (w.nsynth == 0) return;
+  const SphereCore &big = *gs.back();
+  const bool rc64 = kSynthRC[big.tile_synth] == 64;
+  switch (big.precision) {
+    case LMW_FP32: rc64 ? synth_multi<LMW_FP32, 64>(gs, w, stream) : synth_multi<LMW_FP32, 32>(gs, w, stream); break;
+    case LMW_BF16X6: rc64 ? synth_multi<LMW_BF16X6, 64>(gs, w, stream) : synth_multi<LMW_BF16X6, 32>(gs, w, stream); break;
+    default: rc64 ? synth_multi<LMW_BF16X3, 64>(gs, w, stream) : synth_multi<LMW_BF16X3, 32>(gs, w, stream); break;
+  }
+}
+
+template <int PREC, int KC, int NW>
+static void anal_multi(const std::vector<SphereCore *> &gs, const std::vector<double> &dts, int stage, const MultiWork &w, hipStream_t stream) {
+  MultiAnal M{};
+  for (size_t g = 0; g < gs.size(); g++) {
+    SphereCore &c = *gs[g];
+    M.g[g] = AnalGroup{c.geo(), AnalTabs{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p}, c.Gb32.p, c.x0.p, c.xs.p, c.acc.p, dts[g], c.prm.diffusion, c.B};
+  }
+  M.work = w.anal.p;
+  M.nwork = w.nanal;
+  M.stage = stage;
+  const size_t lds = PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float) : nsplit(PREC) * 8 * (KC + 8) * MEM * sizeof(__bf16);
+  const int nblk = persistent_blocks(k_anal_mx_multi<PREC, KC, NW>, 64 * NW, lds, w.nanal);
+  hipLaunchKernelGGL((k_anal_mx_multi<PREC, KC, NW>), dim3(nblk), dim3(64 * NW), lds, stream, M);
+  LMW_LAUNCH_CHECK();
+}
+
+template <int PREC>
+static void anal_multi_dispatch(const std::vector<SphereCore *> &gs, const std::vector<double> &dts, int stage, const MultiWork &w,
+                                hipStream_t stream) {
+  switch (gs.back()->tile_anal) {
+    case 0: anal_multi<PREC, 32, 4>(gs, dts, stage, w, stream); break;
+    case 2: anal_multi<PREC, 128, 4>(gs, dts, stage, w, stream); break;
+    case 3: anal_multi<PREC, 32, 8>(gs, dts, stage, w, stream); break;
+    case 4: anal_multi<PREC, 64, 8>(gs, dts, stage, w, stream); break;
+    case 5: anal_multi<PREC, 128, 8>(gs, dts, stage, w, stream); break;
+    default: anal_multi<PREC, 64, 4>(gs, dts, stage, w, stream); break;
+  }
+}
+
+void launch_anal_mx_multi(const std::vector<SphereCore *> &gs, const std::vector<double> &dts, int stage, const MultiWork &w,
+                          hipStream_t stream) {
+  if (w.nanal == 0) return;
+  switch (gs.back()->precision) {
+    case LMW_FP32: anal_multi_dispatch<LMW_FP32>(gs, dts, stage, w, stream); break;
+    case LMW_BF16X6: anal_multi_dispatch<LMW_BF16X6>(gs, dts, stage, w, stream); break;
+    default: anal_multi_dispatch<LMW_BF16X3>(gs, dts, stage, w, stream); break;
+  }
 }
 
 }  // namespace lmw

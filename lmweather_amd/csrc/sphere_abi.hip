@@ -330,6 +330,11 @@ struct lmw_sphere {
   MemberView view() { return MemberView{core.get(), 0}; }
 };
 
+namespace lmw {
+std::vector<SphereCore *> all_groups(lmw_ensemble *e);
+void drop_macro(lmw_ensemble *e);
+}  // namespace lmw
+
 struct lmw_ensemble {
   std::vector<std::unique_ptr<SphereCore>> groups;  // ascending truncation
   std::vector<std::pair<int, int>> where;           // member -> (group, local index)
@@ -340,6 +345,9 @@ struct lmw_ensemble {
   std::vector<double> macro_dts;
   std::vector<int> macro_sub;
   std::vector<hipEvent_t> events;  // fork, then one join per group
+  // persistent mode: the mixed Legendre transforms of all active groups in one multi-group launch per stage phase
+  bool persistent = false;
+  std::map<unsigned, std::unique_ptr<MultiWork>> work;  // per set of active groups (bit mask)
   ~lmw_ensemble() {
     if (macro) (void)hipGraphExecDestroy(macro);
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
@@ -841,6 +849,75 @@ static void capture_macro(lmw_ensemble *e, const double *dts, const int *substep
   e->macro_dts.assign(dts, dts + ng);
   e->macro_sub.assign(substeps, substeps + ng);
 }
+// Persistent variant of the macro step: ticks of the finest group's step; at each tick the groups due a step run
+// their four stages together, each stage as one multi-group inverse-transform launch, the groups' grid-point
+// kernels, and one multi-group forward-transform launch, all on the origin stream.
+static void capture_macro_persistent(lmw_ensemble *e, const double *dts, const int *substeps) {
+  const size_t ng = e->groups.size();
+  if (ng > static_cast<size_t>(kMaxGroups)) invalid("persistent macro steps serve at most 4 truncation groups");
+  int T = 0;
+  for (size_t i = 0; i < ng; i++) {
+    if (e->groups[i]->precision == LMW_FP64) invalid("persistent macro steps need a mixed-precision ensemble");
+    if (substeps[i] < 1) invalid("persistent macro steps need at least one sub-step per group");
+    T = std::max(T, substeps[i]);
+  }
+  for (size_t i = 0; i < ng; i++)
+    if (T % substeps[i]) invalid("persistent macro steps need sub-step counts that divide the largest one");
+  std::vector<SphereCore *> gs = all_groups(e);
+  std::vector<double> dtv(dts, dts + ng);
+  std::vector<unsigned> masks(T);
+  for (int t = 0; t < T; t++) {
+    unsigned mask = 0;
+    for (size_t i = 0; i < ng; i++)
+      if (t % (T / substeps[i]) == 0) mask |= 1u << i;
+    masks[t] = mask;
+    auto &w = e->work[mask];
+    if (!w) {
+      w = std::make_unique<MultiWork>();
+      build_multi_work(gs, mask, *w);
+    }
+  }
+  if (!e->origin) {
+    LMW_HIP(hipStreamCreate(&e->origin));
+    e->events.resize(ng + 1);
+    for (auto &ev : e->events) LMW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  if (e->macro) {
+    LMW_HIP(hipGraphExecDestroy(e->macro));
+    e->macro = nullptr;
+  }
+  for (auto *c : gs) c->sync();
+  hipGraph_t g;
+  LMW_HIP(hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
+  for (int t = 0; t < T; t++) {
+    const MultiWork &w = *e->work[masks[t]];
+    for (int stage = 1; stage <= 4; stage++) {
+      launch_synth_mx_multi(gs, w, e->origin);
+      // the groups' grid-point kernels (different FFT lengths) as concurrent branches on their own streams
+      LMW_HIP(hipEventRecord(e->events[0], e->origin));
+      for (size_t i = 0; i < ng; i++) {
+        if (!(masks[t] & (1u << i))) continue;
+        LMW_HIP(hipStreamWaitEvent(gs[i]->stream, e->events[0], 0));
+        gs[i]->launch_tend_grid(0, gs[i]->B, 1);
+        LMW_HIP(hipEventRecord(e->events[i + 1], gs[i]->stream));
+        LMW_HIP(hipStreamWaitEvent(e->origin, e->events[i + 1], 0));
+      }
+      launch_anal_mx_multi(gs, dtv, stage, w, e->origin);
+    }
+  }
+  LMW_HIP(hipStreamEndCapture(e->origin, &g));
+  LMW_HIP(hipGraphInstantiate(&e->macro, g, nullptr, nullptr, 0));
+  LMW_HIP(hipGraphDestroy(g));
+  e->macro_dts.assign(dts, dts + ng);
+  e->macro_sub.assign(substeps, substeps + ng);
+}
+lmw_status lmw_ensemble_set_persistent(lmw_ensemble *e, int on) {
+  return guard([&] {
+    NEED(e);
+    if (e->persistent != (on != 0)) drop_macro(e);
+    e->persistent = on != 0;
+  });
+}
 lmw_status lmw_ensemble_macro_steps(lmw_ensemble *e, const double *dts, const int *substeps, int64_t count) {
   return guard([&] {
     NEED(e && dts && substeps);
@@ -855,7 +932,10 @@ lmw_status lmw_ensemble_macro_steps(lmw_ensemble *e, const double *dts, const in
     for (auto &c : e->groups) c->sync();
     bool same = e->macro && e->macro_sub.size() == ng;
     for (size_t i = 0; same && i < ng; i++) same = e->macro_dts[i] == dts[i] && e->macro_sub[i] == substeps[i];
-    if (!same) capture_macro(e, dts, substeps);
+    if (!same) {
+      if (e->persistent) capture_macro_persistent(e, dts, substeps);
+      else capture_macro(e, dts, substeps);
+    }
     for (int64_t k = 0; k < count; k++) LMW_HIP(hipGraphLaunch(e->macro, e->origin));
     LMW_HIP(hipStreamSynchronize(e->origin));
     for (size_t i = 0; i < ng; i++) {

@@ -42,6 +42,7 @@ enum GridMode { GRID_DIAG = 0, GRID_RAW = 1 };
 // Tile variants of the Legendre kernels chosen per truncation group by the autotuner (autotune.hip): FP64
 // members per block; mixed-precision synthesis row chunk; mixed-precision analysis (latitude chunk, waves).
 constexpr int kFp64Tiles = 3, kSynthTiles = 2, kAnalTiles = 6;
+constexpr int kMaxGroups = 4;  // truncation groups served by one multi-group (persistent) Legendre launch
 extern const int kFp64MB[kFp64Tiles];
 extern const int kSynthRC[kSynthTiles];
 extern const int kAnalKC[kAnalTiles], kAnalNW[kAnalTiles];
@@ -150,6 +151,18 @@ struct SphereCore {
 
   size_t packed_index(int m, int n) const { return static_cast<size_t>(h_moff[m] + (n - m)); }
 };
+
+// Persistent multi-group Legendre launches (precision.hip): groups[i] are all the groups of an ensemble (same mixed
+// precision, at most kMaxGroups); work items name them by index; dts[i] is group i's step.  Tiles: those of the last
+// (largest) group.
+struct MultiWork {
+  DevArray<int4> synth, anal;  // (group, m, member tile start, 0) / (group, m, parity, member tile start)
+  int nsynth = 0, nanal = 0;
+};
+void build_multi_work(const std::vector<SphereCore *> &groups, unsigned active_mask, MultiWork &w);
+void launch_synth_mx_multi(const std::vector<SphereCore *> &groups, const MultiWork &w, hipStream_t stream);
+void launch_anal_mx_multi(const std::vector<SphereCore *> &groups, const std::vector<double> &dts, int stage, const MultiWork &w,
+                          hipStream_t stream);
 
 // free functions on the GPU: batched FFT (host arrays in/out) and Gauss-Legendre quadrature
 void fft_batch(double2 *host, int n, int count, int sign);

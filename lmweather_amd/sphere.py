@@ -388,6 +388,10 @@ class Ensemble:
         c = np.ascontiguousarray(substeps, dtype=np.int32)
         L.call("lmw_ensemble_macro_steps", self._h, L.dptr(d), c.ctypes.data_as(L._ip), int(count))
 
+    def set_persistent(self, on: bool) -> None:
+        """Macro steps through persistent multi-group Legendre kernels (mixed precision, at most 4 groups)."""
+        L.call("lmw_ensemble_set_persistent", self._h, int(bool(on)))
+
     def autotune(self, force: bool = False):
         """Choose the Legendre tile shapes of every group (timed, or from the cache); returns (tiles (groups, 4),
         best_ms (groups, 2))."""

@@ -45,6 +45,21 @@ def test_macro_steps_equal_group_steps(lmw, precision):
     assert a.diag(9, lmw._lib.DIAG_TIME) == 1800.0 and a.diag(0, lmw._lib.DIAG_TIME) == 1800.0
 
 
+@pytest.mark.parametrize("precision", [2, 3])
+def test_persistent_macro_steps_equal_branch_macro_steps(lmw, precision):
+    """The persistent multi-group Legendre kernels do per member exactly what the per-group kernels do."""
+    truncs = [42] * 5 + [85] * 3 + [170] * 2
+    a, b = build(lmw, truncs, precision), build(lmw, truncs, precision)
+    a.set_persistent(True)
+    a.macro_steps([600.0, 300.0, 150.0], [1, 2, 4], 3)
+    b.macro_steps([600.0, 300.0, 150.0], [1, 2, 4], 3)
+    for x, y in zip(states(a), states(b)):
+        assert np.array_equal(x, y)
+    assert a.diag(9, lmw._lib.DIAG_TIME) == 1800.0
+    with pytest.raises(ValueError, match="divide the largest"):
+        a.macro_steps([600.0, 200.0, 150.0], [1, 3, 4], 1)
+
+
 def test_every_tile_variant_matches_default(lmw):
     """Mixed BF16x3: each (row chunk) x (latitude chunk, waves) variant stays within the BF16x3 error of the default
     shapes after 4 steps; FP64: the members-per-block variants are bit-identical (members are independent)."""
