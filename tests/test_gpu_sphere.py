@@ -124,6 +124,35 @@ def test_c1_five_days_t42(lmw, oracle):
     assert abs(s.mean_height() - m0r) <= 1e-12 * m0r
 
 
+def test_c2_t170_state_tendencies_and_steps(lmw, oracle):
+    """Config C2's state (T170, balanced flow to n = 85 over 8 seeded mountains, dt = 150 s): set-up, tendencies and
+    8 FP64 steps against the oracle; the BF16x6 Legendre mode's distance from FP64 after the same steps."""
+    from lmweather_amd.initial import gaussian_orography
+    s, o, (h, u, v) = balanced(lmw, oracle, 170, 85, 1)
+    hs = gaussian_orography(OracleTransforms(o, s.params), 8, 2)
+    m = lmw.Sphere(170, s.params)
+    for x in (s, o, m):
+        x.set_orography_grid(hs)
+        x.set_state_grid(h - hs, u, v)
+    m.set_precision(lmw.BF16X6)
+    zscale = np.max(np.abs(o.get_spectral(0)))
+    for f in range(3):
+        assert max_rel(s.get_spectral(f), o.get_spectral(f), zscale if f == 1 else None) <= 1e-12
+    for f, (a, b) in enumerate(zip(s.tendencies(), o.tendencies())):
+        print(f"C2 T170 tendency {f}: rel-L2 {rel_l2(a, b):.2e}, max-rel {max_rel(a, b):.2e}")
+        assert rel_l2(a, b) <= 1e-11 and max_rel(a, b) <= 1e-10
+    s.steps(150.0, 8)
+    m.steps(150.0, 8)
+    for _ in range(8):
+        o.step(150.0)
+    err = rel_l2(s.grid_height(), o.grid_height())
+    emx = rel_l2(m.grid_height(), o.grid_height())
+    print(f"C2 T170, 8 steps: FP64 rel-L2(h) vs oracle {err:.2e}; BF16x6 {emx:.2e}")
+    assert err <= 1e-12
+    assert emx <= 1e-8
+    assert m.mean_height() == s.mean_height()
+
+
 def test_diagnostics_and_errors(lmw, oracle):
     s, o, (h, u, v) = balanced(lmw, oracle, 42, 21, 7)
     s.set_state_grid(h, u, v)
