@@ -297,6 +297,9 @@ lmw_status lmw_plane_run(lmw_plane *p, double seconds, double dt);
 lmw_status lmw_plane_stable_dt(lmw_plane *p, double safety, double *out);
 /* mean_depth/energy/max_speed/potential_vorticity_anomaly_rms/time: cpp/plane.hpp:45-48,25 / plane.cpp:191-243 */
 lmw_status lmw_plane_diag(lmw_plane *p, int member, int which, double *out);
+/* the plane's step kernel (no reference counterpart): the fused RK4 step with its four stages in LDS on tiles of
+ * 0 = 32 x 32 cells (default), 1 = 32 x 16, 2 = 64 x 32 (FP32 only), or 3 = four stage launches; all bit-identical */
+lmw_status lmw_plane_set_tile(lmw_plane *p, int tile);
 /* one evaluation of tendency(h, u, v) for one member: cpp/plane.cpp:59-129 */
 lmw_status lmw_plane_tendency(lmw_plane *p, int member, double *dh, double *du, double *dv);
 

@@ -140,6 +140,7 @@ SIGNATURES = {
     "lmw_plane_stable_dt": [_vp, C.c_double, _dp],
     "lmw_plane_diag": [_vp, C.c_int, C.c_int, _dp],
     "lmw_plane_tendency": [_vp, C.c_int, _dp, _dp, _dp],
+    "lmw_plane_set_tile": [_vp, C.c_int],
 }
 
 _lib = None

@@ -2,11 +2,12 @@
 // members that share the parameters, in FP64 or FP32.
 //
 // Layout: per member h[ny][nx], u[ny][nx] (west faces), v[ny+1][nx] (south faces; rows 0 and ny are walls, zero),
-// members contiguous.  One kernel per RK4 stage computes the whole tendency of a cell (Bernoulli function at
-// centres, absolute vorticity at corners, du, dv, dh: plane.cpp:59-129) from its 3x3 neighbourhood, recomputing
-// B and eta at the few points it needs instead of storing them, and applies the RK4 stage update (plane.cpp:143-158)
-// in the same pass.  A block is a 64 x 4 tile of cells; neighbour reuse goes through the L1/L2 caches, so HBM sees
-// each field once per stage.  Buffers x0 (state), xa, xb (stage inputs, ping-pong) and acc (RK4 sum).
+// members contiguous.  The step path is k_plane_step: one launch per RK4 step, each block loading a tile plus a
+// 4-cell halo into LDS, running the four stages there (Bernoulli function at centres and absolute vorticity at
+// corners once per stage, then du, dv, dh per cell: plane.cpp:59-129, and the stage update: plane.cpp:143-158), and
+// writing its interior back, so HBM sees the state once in and once out per step.  k_plane_stage, one launch per
+// stage with a 64 x 4 tile per block and buffers x0 (state), xa, xb (stage inputs, ping-pong) and acc (RK4 sum),
+// computes the same arithmetic in the same order and is kept as the reference shape for the fused kernel.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -23,44 +24,90 @@ struct PlaneGeo {
 
 namespace {
 
-// Tendency of cell (i, j) of one member (plane.cpp:59-129), arithmetic in T.
-template <class T>
-__device__ inline void cell_tendency(const PlaneGeo &P, const T *__restrict__ h, const T *__restrict__ u, const T *__restrict__ v, int i,
-                                     int j, T &dh, T &du, T &dv) {
+// Field access for cell_tendency: h, u, v at (x index i, global row j) and the x neighbours of i.
+// Global arrays of one member: periodic in x.
+template <class T> struct GridAccess {
+  const T *h, *u, *v;
+  int nx;
+  __device__ T H(int i, int j) const { return h[static_cast<size_t>(j) * nx + i]; }
+  __device__ T U(int i, int j) const { return u[static_cast<size_t>(j) * nx + i]; }
+  __device__ T V(int i, int j) const { return v[static_cast<size_t>(j) * nx + i]; }
+  __device__ int xm(int i) const { return i == 0 ? nx - 1 : i - 1; }
+  __device__ int xp(int i) const { return i == nx - 1 ? 0 : i + 1; }
+};
+// An LDS tile of EX columns and rows jb .. jb + EY - 1 (v: jb .. jb + EY): neighbours outside the tile are clamped
+// to its edge, which only affects cells whose results the caller discards.
+template <class T> struct TileAccess {
+  const T *h, *u, *v;
+  int ex, ey, jb;
+  __device__ int row(int j, int last) const { return min(max(j - jb, 0), last); }
+  __device__ T H(int i, int j) const { return h[row(j, ey - 1) * ex + i]; }
+  __device__ T U(int i, int j) const { return u[row(j, ey - 1) * ex + i]; }
+  __device__ T V(int i, int j) const { return v[row(j, ey) * ex + i]; }
+  __device__ int xm(int i) const { return i > 0 ? i - 1 : 0; }
+  __device__ int xp(int i) const { return i < ex - 1 ? i + 1 : ex - 1; }
+};
+
+// B at the centres and eta at the corners of an LDS tile (rows jb .. jb + EY - 1, eta rows to jb + EY)
+template <class T> struct TileBE {
+  const T *b, *eta;
+  int ex, ey, jb;
+  __device__ T B(int i, int j) const { return b[min(max(j - jb, 0), ey - 1) * ex + i]; }
+  __device__ T Eta(int i, int j) const { return eta[min(max(j - jb, 0), ey) * ex + i]; }
+};
+
+// Bernoulli function B = g h + K at centre (ii, jj) (plane.cpp:72-77)
+template <class T, class A> __device__ inline T bernoulli(const PlaneGeo &P, const A &F, int ii, int jj) {
 #pragma clang fp contract(off)
-  const int nx = P.nx, ny = P.ny;
-  const T dxx = static_cast<T>(P.dx), dyy = static_cast<T>(P.dy), g = static_cast<T>(P.g);
+  const T half = static_cast<T>(0.5), g = static_cast<T>(P.g);
+  const int i1 = F.xp(ii);
+  T k = 0;
+  if (P.linear == 0)
+    k = half * (half * (F.U(ii, jj) * F.U(ii, jj) + F.U(i1, jj) * F.U(i1, jj)) + half * (F.V(ii, jj) * F.V(ii, jj) + F.V(ii, jj + 1) * F.V(ii, jj + 1)));
+  return g * F.H(ii, jj) + k;
+}
+// absolute vorticity at corner (ii, jj), jj = 0..ny (plane.cpp:80-92)
+template <class T, class A> __device__ inline T abs_vorticity(const PlaneGeo &P, const A &F, int ii, int jj) {
+#pragma clang fp contract(off)
+  const int ny = P.ny;
+  const T dxx = static_cast<T>(P.dx), dyy = static_cast<T>(P.dy);
+  const T f = static_cast<T>(P.f0 + P.beta * (jj * P.dy - 0.5 * P.ly));
+  T zeta = 0;
+  if (P.linear == 0) {
+    const int i0 = F.xm(ii);
+    const T dvdx = (F.V(ii, jj) - F.V(i0, jj)) / dxx;
+    const T uj = jj < ny ? F.U(ii, jj) : F.U(ii, ny - 1), ujm = jj > 0 ? F.U(ii, jj - 1) : F.U(ii, 0);
+    zeta = dvdx - (uj - ujm) / dyy;
+  }
+  return f + zeta;
+}
+// B and eta computed where they are asked for
+template <class T, class A> struct OnTheFly {
+  const PlaneGeo &P;
+  const A &F;
+  __device__ T B(int ii, int jj) const { return bernoulli<T>(P, F, ii, jj); }
+  __device__ T Eta(int ii, int jj) const { return abs_vorticity<T>(P, F, ii, jj); }
+};
+
+// Tendency of cell (i, j) of one member (plane.cpp:59-129), arithmetic in T; j is the global row (wall logic); E
+// supplies the Bernoulli function at centres and the absolute vorticity at corners.
+template <class T, class A, class E>
+__device__ inline void cell_tendency(const PlaneGeo &P, const A &F, const E &Q, int i, int j, T &dh, T &du, T &dv) {
+#pragma clang fp contract(off)
+  const int ny = P.ny;
+  const T dxx = static_cast<T>(P.dx), dyy = static_cast<T>(P.dy);
   const T half = static_cast<T>(0.5), quarter = static_cast<T>(0.25), two = static_cast<T>(2);
-  const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-  auto H = [&](int ii, int jj) { return h[static_cast<size_t>(jj) * nx + ii]; };
-  auto U = [&](int ii, int jj) { return u[static_cast<size_t>(jj) * nx + ii]; };
-  auto V = [&](int ii, int jj) { return v[static_cast<size_t>(jj) * nx + ii]; };
+  const int im = F.xm(i), ip = F.xp(i);
+  auto H = [&](int ii, int jj) { return F.H(ii, jj); };
+  auto U = [&](int ii, int jj) { return F.U(ii, jj); };
+  auto V = [&](int ii, int jj) { return F.V(ii, jj); };
   const bool lin = P.linear != 0;
-  // Bernoulli function B = g h + K at a centre (plane.cpp:72-77); ii already wrapped
-  auto Bf = [&](int ii, int jj) {
-    const int i1 = ii == nx - 1 ? 0 : ii + 1;
-    T k = 0;
-    if (!lin) k = half * (half * (U(ii, jj) * U(ii, jj) + U(i1, jj) * U(i1, jj)) + half * (V(ii, jj) * V(ii, jj) + V(ii, jj + 1) * V(ii, jj + 1)));
-    return g * H(ii, jj) + k;
-  };
-  // absolute vorticity at corner (ii, jj), jj = 0..ny (plane.cpp:80-92); ii already wrapped
-  auto Eta = [&](int ii, int jj) {
-    const T f = static_cast<T>(P.f0 + P.beta * (jj * P.dy - 0.5 * P.ly));
-    T zeta = 0;
-    if (!lin) {
-      const int i0 = ii == 0 ? nx - 1 : ii - 1;
-      const T dvdx = (V(ii, jj) - V(i0, jj)) / dxx;
-      const T uj = jj < ny ? U(ii, jj) : U(ii, ny - 1), ujm = jj > 0 ? U(ii, jj - 1) : U(ii, 0);
-      zeta = dvdx - (uj - ujm) / dyy;
-    }
-    return f + zeta;
-  };
   const T drag = static_cast<T>(P.drag), visc = static_cast<T>(P.visc), depth = static_cast<T>(P.depth);
   // du at the u point (i, j): west face (plane.cpp:94-105)
   {
-    const T q = half * (Eta(i, j) + Eta(i, j + 1));
+    const T q = half * (Q.Eta(i, j) + Q.Eta(i, j + 1));
     const T vbar = quarter * (V(im, j) + V(i, j) + V(im, j + 1) + V(i, j + 1));
-    T r = q * vbar - (Bf(i, j) - Bf(im, j)) / dxx - drag * U(i, j);
+    T r = q * vbar - (Q.B(i, j) - Q.B(im, j)) / dxx - drag * U(i, j);
     if (P.visc != 0) {
       const T uy1 = j + 1 < ny ? U(i, j + 1) : U(i, j), uy0 = j > 0 ? U(i, j - 1) : U(i, j);
       r += visc * ((U(ip, j) - two * U(i, j) + U(im, j)) / (dxx * dxx) + (uy1 - two * U(i, j) + uy0) / (dyy * dyy));
@@ -69,9 +116,9 @@ __device__ inline void cell_tendency(const PlaneGeo &P, const T *__restrict__ h,
   }
   // dv at the v point (i, j): south face, rows 1..ny-1 (plane.cpp:106-117)
   if (j >= 1) {
-    const T q = half * (Eta(i, j) + Eta(ip, j));
+    const T q = half * (Q.Eta(i, j) + Q.Eta(ip, j));
     const T ubar = quarter * (U(i, j - 1) + U(ip, j - 1) + U(i, j) + U(ip, j));
-    T r = -q * ubar - (Bf(i, j) - Bf(i, j - 1)) / dyy - drag * V(i, j);
+    T r = -q * ubar - (Q.B(i, j) - Q.B(i, j - 1)) / dyy - drag * V(i, j);
     if (P.visc != 0) {
       const T vy1 = j + 1 < ny ? V(i, j + 1) : T(0), vy0 = j - 1 > 0 ? V(i, j - 1) : T(0);
       r += visc * ((V(ip, j) - two * V(i, j) + V(im, j)) / (dxx * dxx) + (vy1 - two * V(i, j) + vy0) / (dyy * dyy));
@@ -90,6 +137,14 @@ __device__ inline void cell_tendency(const PlaneGeo &P, const T *__restrict__ h,
     if (j > 0) fs = (lin ? depth : half * (H(i, j - 1) + H(i, j))) * V(i, j);
     dh = -((fe - fw) / dxx + (fn - fs) / dyy);
   }
+}
+template <class T, class A> __device__ inline void cell_tendency(const PlaneGeo &P, const A &F, int i, int j, T &dh, T &du, T &dv) {
+  cell_tendency<T>(P, F, OnTheFly<T, A>{P, F}, i, j, dh, du, dv);
+}
+template <class T>
+__device__ inline void cell_tendency(const PlaneGeo &P, const T *__restrict__ h, const T *__restrict__ u, const T *__restrict__ v, int i,
+                                     int j, T &dh, T &du, T &dv) {
+  cell_tendency<T>(P, GridAccess<T>{h, u, v, P.nx}, i, j, dh, du, dv);
 }
 
 // One RK4 stage for all members: tendency of `in`, then the stage update (plane.cpp:143-158).
@@ -129,6 +184,87 @@ __global__ __launch_bounds__(256) void k_plane_stage(PlaneGeo P, int stage, doub
     const T s0 = x0[q];
     if (stage == 1) acc[q] = 0;
     if (stage <= 3) out[q] = s0;
+  }
+}
+
+// One whole RK4 step (plane.cpp:143-158) of a TX x TY tile per block of NT threads, its four stages in LDS: the
+// block loads the tile and a halo of 4 cells (the stencil reaches one cell per stage) from `in`, runs the stages on
+// the extended tile (each stage's results are exact one cell further inside than the previous stage's; the clamped
+// values at the tile edge only reach cells that are discarded), and writes the interior to `out`.  The stage inputs
+// ping-pong between two LDS copies; the RK4 sums of the interior cells stay in registers.  Arithmetic per cell is
+// that of k_plane_stage in the same order, so the step is bit-identical to four stage launches, at one read of the
+// extended tile and one write of the interior per step.
+template <class T, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_plane_step(PlaneGeo P, double dt, const T *__restrict__ in, T *__restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int HALO = 4, EX = TX + 2 * HALO, EY = TY + 2 * HALO, NC = EX * EY, NV = EX * (EY + 1), NS = 2 * NC + NV;
+  constexpr int PI = TX * TY / NT;  // interior cells per thread
+  static_assert(TX * TY % NT == 0, "the interior must divide among the threads");
+  extern __shared__ double plane_lds[];
+  T *X = reinterpret_cast<T *>(plane_lds), *S0 = X + NS, *S1 = S0 + NS;  // state at the start, two stage inputs
+  T *Bc = S1 + NS, *Ec = Bc + NC;  // Bernoulli function at the centres, absolute vorticity at the corners
+  const int nx = P.nx, ny = P.ny, b = blockIdx.z, tid = threadIdx.x;
+  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, ib = i0 - HALO, jb = j0 - HALO;
+  const size_t nh = static_cast<size_t>(nx) * ny, nvv = static_cast<size_t>(nx) * (ny + 1), per = 2 * nh + nvv;
+  const T *gh = in + b * per, *gu = gh + nh, *gv = gu + nh;
+  // load (periodic in x); rows outside the domain are never read by a cell inside it: zeros
+  for (int e = tid; e < NV; e += NT) {
+    const int ex = e % EX, ey = e / EX, j = jb + ey, i = ((ib + ex) % nx + nx) % nx;
+    if (ey < EY) {
+      const bool dom = j >= 0 && j < ny;
+      const T hv = dom ? gh[static_cast<size_t>(j) * nx + i] : T(0), uv = dom ? gu[static_cast<size_t>(j) * nx + i] : T(0);
+      X[e] = S0[e] = hv;
+      X[NC + e] = S0[NC + e] = uv;
+    }
+    const T vv = (j >= 0 && j <= ny) ? gv[static_cast<size_t>(j) * nx + i] : T(0);
+    X[2 * NC + e] = S0[2 * NC + e] = S1[2 * NC + e] = vv;  // S1's top v row is never rewritten: it keeps its value
+  }
+  __syncthreads();
+  const T hdt = static_cast<T>(0.5 * dt), fdt = static_cast<T>(dt), sdt = static_cast<T>(dt / 6), two = static_cast<T>(2);
+  T acc[PI][3];
+  for (int stage = 1; stage <= 4; stage++) {
+    const T *cur = (stage & 1) ? S0 : S1;
+    T *nxt = (stage & 1) ? S1 : S0;
+    const TileAccess<T> F{cur, cur + NC, cur + 2 * NC, EX, EY, jb};
+    const T c = stage == 3 ? fdt : hdt;
+    // B at every centre and eta at every corner of the tile, once (plane.cpp:72-92)
+    for (int e = tid; e < NV; e += NT) {
+      const int ex = e % EX, ey = e / EX, j = jb + ey;
+      if (ey < EY) Bc[e] = (j >= 0 && j < ny) ? bernoulli<T>(P, F, ex, j) : T(0);
+      Ec[e] = (j >= 0 && j <= ny) ? abs_vorticity<T>(P, F, ex, j) : T(0);
+    }
+    __syncthreads();
+    const TileBE<T> Q{Bc, Ec, EX, EY, jb};
+    // interior cells: the RK4 sum, then the next stage input or the result
+#pragma unroll
+    for (int o = 0; o < PI; o++) {
+      const int p = tid + NT * o, ix = p % TX, iy = p / TX, e = (iy + HALO) * EX + ix + HALO, j = j0 + iy;
+      T k[3] = {T(0), T(0), T(0)};
+      if (j < ny) cell_tendency<T>(P, F, Q, ix + HALO, j, k[0], k[1], k[2]);
+      for (int f = 0; f < 3; f++) acc[o][f] = stage == 1 ? k[f] : (stage <= 3 ? acc[o][f] + two * k[f] : acc[o][f]);
+      if (stage <= 3) {
+        for (int f = 0; f < 3; f++) nxt[f * NC + e] = X[f * NC + e] + c * k[f];
+      } else if (j < ny && i0 + ix < nx) {
+        T *oh = out + b * per, *ou = oh + nh, *ov = ou + nh;
+        const size_t q = static_cast<size_t>(j) * nx + i0 + ix;
+        oh[q] = X[e] + sdt * (acc[o][0] + k[0]);
+        ou[q] = X[NC + e] + sdt * (acc[o][1] + k[1]);
+        ov[q] = X[2 * NC + e] + sdt * (acc[o][2] + k[2]);
+        if (j == ny - 1) ov[static_cast<size_t>(ny) * nx + i0 + ix] = X[2 * NC + e + EX];  // the northern wall row of v
+      }
+    }
+    // the halo ring: next stage input only
+    if (stage <= 3) {
+      for (int e = tid; e < NC; e += NT) {
+        const int ex = e % EX, ey = e / EX;
+        if (ex >= HALO && ex < HALO + TX && ey >= HALO && ey < HALO + TY) continue;
+        const int j = jb + ey;
+        T k[3] = {T(0), T(0), T(0)};
+        if (j >= 0 && j < ny) cell_tendency<T>(P, F, Q, ex, j, k[0], k[1], k[2]);
+        for (int f = 0; f < 3; f++) nxt[f * NC + e] = X[f * NC + e] + c * k[f];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -265,6 +401,7 @@ struct PlaneCoreBase {
   virtual void steps(double dt, int64_t count) = 0;
   virtual void reduce(double *out8B) = 0;  // B x 8
   virtual void tendency(int b, double *dh, double *du, double *dv) = 0;
+  virtual void set_tile(int tile) = 0;
   virtual void reset() = 0;
   void sync() { LMW_HIP(hipStreamSynchronize(stream)); }
 };
@@ -323,7 +460,35 @@ template <class T> struct PlaneCore : PlaneCoreBase {
     hipLaunchKernelGGL(k_plane_stage<T>, grd, blk, 0, stream, geo(), stage, dt, in, x0.p, out, acc.p);
     LMW_LAUNCH_CHECK();
   }
+  // whole RK4 steps, one fused launch each (k_plane_step), the state ping-ponging between x0 and xa
+  int tile = 0;  // fused: 0 = 32 x 32 interior, 512 threads; 1 = 32 x 16, 256 threads; 2 = 64 x 32, 512 threads; 3 = staged
+  template <int TX, int TY, int NT> void launch_step(double dt) {
+    constexpr int EX = TX + 8, EY = TY + 8;
+    const size_t lds = (3 * (2 * static_cast<size_t>(EX) * EY + static_cast<size_t>(EX) * (EY + 1)) + static_cast<size_t>(EX) * EY +
+                        static_cast<size_t>(EX) * (EY + 1)) * sizeof(T);
+    dim3 grd((prm.nx + TX - 1) / TX, (prm.ny + TY - 1) / TY, B), blk(NT);
+    hipLaunchKernelGGL((k_plane_step<T, TX, TY, NT>), grd, blk, lds, stream, geo(), dt, x0.p, xa.p);
+    LMW_LAUNCH_CHECK();
+    std::swap(x0.p, xa.p);
+  }
+  void set_tile(int t) override {
+    if (t < 0 || t > 3) invalid("plane tile must be 0, 1, 2 or 3");
+    if (t == 2 && sizeof(T) == 8) invalid("the 64 x 32 plane tile needs more LDS than a workgroup has in FP64");
+    tile = t;
+  }
   void steps(double dt, int64_t count) override {
+    if (tile == 3) {
+      steps_staged(dt, count);
+      return;
+    }
+    for (int64_t s = 0; s < count; s++) {
+      if (tile == 1) launch_step<32, 16, 256>(dt);
+      else if (tile == 2) launch_step<64, 32, 512>(dt);
+      else launch_step<32, 32, 512>(dt);
+    }
+  }
+  // the same steps as four stage launches through x0, acc and the stage buffers
+  void steps_staged(double dt, int64_t count) {
     for (int64_t s = 0; s < count; s++) {
       launch_stage(1, dt, x0.p, xa.p);
       launch_stage(2, dt, xa.p, xb.p);
@@ -414,6 +579,9 @@ lmw_status lmw_plane_create(const lmw_plane_params *params, int batch, int preci
     h->precision = precision;
     *out = h.release();
   });
+}
+lmw_status lmw_plane_set_tile(lmw_plane *p, int tile) {
+  return guard([&] { P(p).core->set_tile(tile); });
 }
 lmw_status lmw_plane_destroy(lmw_plane *p) {
   return guard([&] { delete p; });

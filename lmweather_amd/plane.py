@@ -90,6 +90,11 @@ class Plane:
     def step(self, dt: float) -> None:
         L.call("lmw_plane_step", self._h, dt)
 
+    def set_tile(self, tile: int) -> None:
+        """Step kernel: the fused RK4 step on tiles 0 = 32 x 32 (default), 1 = 32 x 16, 2 = 64 x 32 (FP32 only), or
+        3 = four stage launches; all bit-identical."""
+        L.call("lmw_plane_set_tile", self._h, int(tile))
+
     def steps(self, dt: float, count: int) -> None:
         L.call("lmw_plane_steps", self._h, dt, count)
 
