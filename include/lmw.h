@@ -291,14 +291,18 @@ lmw_status lmw_plane_get_field(lmw_plane *p, int member, int field, double *out,
 /* step(dt): cpp/plane.hpp:41 / plane.cpp:143-158 (all members) */
 lmw_status lmw_plane_step(lmw_plane *p, double dt);
 lmw_status lmw_plane_steps(lmw_plane *p, double dt, int64_t count);
+/* lmw_plane_steps, also reporting the time of the step kernels alone (HIP events on the plane's stream) in
+ * *kernel_ms (no reference counterpart: instrumentation for the benchmark) */
+lmw_status lmw_plane_steps_timed(lmw_plane *p, double dt, int64_t count, double *kernel_ms);
 /* run(seconds, dt): cpp/plane.hpp:42 / plane.cpp:160-189 (all members; the limit is the smallest over members) */
 lmw_status lmw_plane_run(lmw_plane *p, double seconds, double dt);
 /* stable_dt(safety): cpp/plane.hpp:43 / plane.cpp:131-141 (the smallest over members) */
 lmw_status lmw_plane_stable_dt(lmw_plane *p, double safety, double *out);
 /* mean_depth/energy/max_speed/potential_vorticity_anomaly_rms/time: cpp/plane.hpp:45-48,25 / plane.cpp:191-243 */
 lmw_status lmw_plane_diag(lmw_plane *p, int member, int which, double *out);
-/* the plane's step kernel (no reference counterpart): the fused RK4 step with its four stages in LDS on tiles of
- * 0 = 32 x 32 cells (default), 1 = 32 x 16, 2 = 64 x 32 (FP32 only), or 3 = four stage launches; all bit-identical */
+/* the plane's step kernel (no reference counterpart): the row march, one launch per RK4 step with each wavefront
+ * walking a 56-column strip in registers, with rows per wavefront chosen from the occupancy (0, default) or
+ * 1 = 32, 2 = 64, 3 = 128, 4 = 256 rows; or 5 = four stage launches; all bit-identical */
 lmw_status lmw_plane_set_tile(lmw_plane *p, int tile);
 /* one evaluation of tendency(h, u, v) for one member: cpp/plane.cpp:59-129 */
 lmw_status lmw_plane_tendency(lmw_plane *p, int member, double *dh, double *du, double *dv);

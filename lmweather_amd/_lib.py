@@ -136,6 +136,7 @@ SIGNATURES = {
     "lmw_plane_get_field": [_vp, C.c_int, C.c_int, _dp, C.c_size_t],
     "lmw_plane_step": [_vp, C.c_double],
     "lmw_plane_steps": [_vp, C.c_double, C.c_int64],
+    "lmw_plane_steps_timed": [_vp, C.c_double, C.c_int64, _dp],
     "lmw_plane_run": [_vp, C.c_double, C.c_double],
     "lmw_plane_stable_dt": [_vp, C.c_double, _dp],
     "lmw_plane_diag": [_vp, C.c_int, C.c_int, _dp],

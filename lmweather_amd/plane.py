@@ -91,12 +91,18 @@ class Plane:
         L.call("lmw_plane_step", self._h, dt)
 
     def set_tile(self, tile: int) -> None:
-        """Step kernel: the fused RK4 step on tiles 0 = 32 x 32 (default), 1 = 32 x 16, 2 = 64 x 32 (FP32 only), or
-        3 = four stage launches; all bit-identical."""
+        """Step kernel: the row march (one launch per RK4 step) with rows per wavefront chosen from the occupancy
+        (0, the default) or 1 = 32, 2 = 64, 3 = 128, 4 = 256 rows; or 5 = four stage launches. All bit-identical."""
         L.call("lmw_plane_set_tile", self._h, int(tile))
 
     def steps(self, dt: float, count: int) -> None:
         L.call("lmw_plane_steps", self._h, dt, count)
+
+    def steps_timed(self, dt: float, count: int) -> float:
+        """steps(dt, count), returning the time of the step kernels alone in ms (HIP events on the plane's stream)."""
+        ms = C.c_double()
+        L.call("lmw_plane_steps_timed", self._h, dt, count, C.byref(ms))
+        return ms.value
 
     def run(self, seconds: float, dt: float) -> None:
         L.call("lmw_plane_run", self._h, seconds, dt)

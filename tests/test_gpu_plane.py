@@ -116,12 +116,16 @@ def test_run_matches_oracle(lmw, oracle):
     assert max_rel(g.get_field(0), o.get_field(0)) <= 1e-13
 
 
-@pytest.mark.parametrize("precision,tiles", [(0, (0, 1, 3)), (1, (0, 1, 2, 3))])
-def test_fused_step_tiles_bit_identical(lmw, oracle, precision, tiles):
-    """The fused RK4 step (four stages per launch in LDS) on every tile shape: bit-identical results; FP64 equal to the
-    oracle to the last bits on a grid whose size is not a multiple of the tiles (partial and wrapped tiles)."""
+@pytest.mark.parametrize("precision,tiles,physics", [(0, (5, 0, 1, 2, 4), "viscous"), (1, (5, 0, 1, 3), "viscous"),
+                                                     (0, (5, 0, 1), "linear"), (0, (5, 0, 1, 4), "inviscid"),
+                                                     (1, (5, 0, 1, 2), "inviscid")])
+def test_fused_step_tiles_bit_identical(lmw, oracle, precision, tiles, physics):
+    """The row march (one launch per RK4 step) in chunks of 32 to 256 rows against four stage launches: bit-identical
+    results; FP64 equal to the oracle to the last bits on a grid whose width is not a multiple of the 56-column strips
+    (a partial strip that wraps around) and whose 37 rows split into 32-row chunks (the second starts 4 rows early)."""
     nx, ny = 100, 37
-    p = lmw.PlaneParams(nx, ny, 1.0e6, 3.7e5, 9.81, 1000.0, 1e-4, 1.6e-11, 50.0, 1e-6)
+    visc, drag, linear = {"viscous": (50.0, 1e-6, False), "linear": (0.0, 0.0, True), "inviscid": (0.0, 1e-6, False)}[physics]
+    p = lmw.PlaneParams(nx, ny, 1.0e6, 3.7e5, 9.81, 1000.0, 1e-4, 1.6e-11, visc, drag, linear)
     rng = np.random.default_rng(5)
     h = 1000.0 + rng.standard_normal((ny, nx))
     u = 0.5 * rng.standard_normal((ny, nx))
@@ -137,11 +141,11 @@ def test_fused_step_tiles_bit_identical(lmw, oracle, precision, tiles):
             g.set_field(2, v, m)
         g.steps(20.0, 7)
         outs.append([g.get_field(f, 1) for f in range(3)])
-    for o in outs[1:]:
-        for a, b in zip(o, outs[0]):
-            assert np.array_equal(a, b)
+    for t, o in zip(tiles[1:], outs[1:]):
+        for f, (a, b) in enumerate(zip(o, outs[0])):
+            assert np.array_equal(a, b), (t, f, np.max(np.abs(a - b)))
     if precision == 0:
-        q = oracle.OraclePlane(nx, ny, 1.0e6, 3.7e5, 9.81, 1000.0, 1e-4, 1.6e-11, 50.0, 1e-6)
+        q = oracle.OraclePlane(nx, ny, 1.0e6, 3.7e5, 9.81, 1000.0, 1e-4, 1.6e-11, visc, drag, linear)
         q.set_field(0, h + 1)
         q.set_field(1, u)
         q.set_field(2, v)
