@@ -5,11 +5,12 @@ a = 6.371e6 m, dt = 150 s) -- the T170 share of the 256-member uneven ensemble -
 (energy up to n = 85, seed 1) over 8 seeded Gaussian mountains, with its vorticity coefficients perturbed by 1e-3
 relative noise (seed 1000 + member).  Inputs are resident in HBM before the timed region.
 
-value = whole-job member-days/s of the batch with the Legendre contractions in ``--precision`` (default split BF16
-hi+lo on MFMA); the same batch in FP64 and one FP64 member ("plain") are timed beside it.  The accuracy of the mixed
+value = whole-job member-days/s of the batch with the Legendre contractions in ``--precision`` (default BF16x6:
+FP32 operands split into three BF16 parts, six MFMA products, FP32 accumulation); the same batch in FP64 and one FP64 member ("plain") are timed beside it.  The accuracy of the mixed
 precision is the rel-L2 of h, u, v against FP64 after ``--check-days`` simulated days of one member, next to an FP64
 twin started from a 1e-7 relative vorticity perturbation (the dynamics' own error growth).  The roofline is that of
-the dominant kernel, measured with HIP events around every launch on the library's stream.  The reference CPU code
+the dominant kernel, measured with HIP events around every launch on the library's stream in a second pass of as
+many steps right after the timed region.  The reference CPU code
 (oracle/_ref, one core) is timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -82,10 +83,15 @@ def kernel_costs(J, K, S, M1, members, mixed):
 
 
 def timed_batch(ens, steps, warmup):
+    """Wall time of `steps` RK4 steps on the production path (one HIP graph replay per step; synchronous), then the
+    per-kernel times of as many steps again with HIP events around every kernel on the library's stream (the events
+    add a few microseconds between kernels, so they stay out of the timed region)."""
     ens.steps([DT], [warmup])
     t0 = time.perf_counter()
-    ms = ens.steps_timed([DT], [steps])[0]  # synchronous: returns once the device has finished
-    return time.perf_counter() - t0, ms
+    ens.steps([DT], [steps])
+    wall = time.perf_counter() - t0
+    ms = ens.steps_timed([DT], [steps])[0]
+    return wall, ms
 
 
 def cpu_baseline(state, steps):

@@ -238,7 +238,8 @@ __global__ __launch_bounds__(64 * MB) void k_synth(Geo G, const double *__restri
 // mflags[b]: bit 0 = rotation axis untilted (f = 2 Omega mu computed, not read), bit 1 = no orography (hs = 0).
 // Grid-point part of one RK4 stage for one latitude pair and member: inverse FFTs of U, V, zeta, phi (north and
 // south rows), the nonlinear products (sphere.cpp:348-358), forward FFTs of the five products and the analysis
-// operands.  TS is the storage type of the Fourier rows and operands; the arithmetic runs in its precision
+// operands.  All global loads a thread needs (coefficients, twiddles, orography, f) are issued before the first
+// result is used.  TS is the storage type of the Fourier rows and operands; the arithmetic runs in its precision
 // (double2: the FP64 step; float2: the mixed-precision step, whose rows hold FP32 values already).
 template <int R1, int R2, class TS>
 __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__ Fs, const double *__restrict__ fgrid,
@@ -249,50 +250,94 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   using C = TS;
   using T = typename real_of<C>::type;
   constexpr int L = F::L, P = F::P;
+  constexpr int NIT = (2 * L + 255) / 256;  // blockDim.x == 256
   extern __shared__ double2 lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
   const int K = 2 * L, M1 = G.M1, tid = threadIdx.x;
   const int jh = blockIdx.x, b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
   C *buf = lds, *tw = lds + 10 * P;
-  // mixed precision: the phi rows hold phi' = phi - phibar (phibar = phi_00 / sqrt 2, the area mean), so the phi
-  // fluxes and the Bernoulli function are formed with phi'; the forward transform's epilogue adds the mean's exact
-  // contribution -phibar delta (the mean's part of the Bernoulli function only reaches n = 0, whose tendency is 0)
-  for (int i = tid; i < K; i += blockDim.x) tw[i] = to_c<C>(twg[i]);
-  __syncthreads();
-  for (int idx = tid; idx < 8 * L; idx += blockDim.x) {
-    const int r = idx / L, mp = idx - r * L;
-    const int j = (r & 1) ? js : jn;
-    const TS *src = Fs + (static_cast<size_t>(b) * G.J + j) * M1 * 4 + (r >> 1);
-    buf[r * P + F::phys(mp)] = irfft_entry(src, mp, L, G.N, tw);
+  const int k = tid;  // one pair of longitudes (2k, 2k+1) per thread in the grid products: L <= 256
+  const int flags = mflags[b];
+  const bool untilted = flags & 1, flat = flags & 2;
+  // orography and f at this thread's longitudes of both rows
+  T hsv[2][2], fv[2][2];
+#pragma unroll
+  for (int s = 0; s < 2; s++) {
+    const size_t gi = (static_cast<size_t>(b) * G.J + (s ? js : jn)) * K + 2 * k;
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      hsv[s][e] = fv[s][e] = T(0);
+      if (k < L && !flat) hsv[s][e] = static_cast<T>(hs[gi + e]);
+      if (k < L && !untilted) fv[s][e] = static_cast<T>(fgrid[gi + e]);
+    }
+  }
+  // inverse real FFT inputs (the arithmetic of irfft_entry): per (row s, m') the four fields' coefficients at m'
+  // and L - m', 4 contiguous complex values each
+  {
+    TS lo[NIT][4], hi[NIT][4];
+    C twm[NIT];
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+      const int it = tid + 256 * q, s = it / L, mp = it - s * L;
+#pragma unroll
+      for (int f = 0; f < 4; f++) lo[q][f] = hi[q][f] = TS{0, 0};
+      twm[q] = mkc<C>(0, 0);
+      if (it < 2 * L) {
+        const TS *row = Fs + (static_cast<size_t>(b) * G.J + (s ? js : jn)) * M1 * 4;
+        if (mp <= G.N) {
+#pragma unroll
+          for (int f = 0; f < 4; f++) lo[q][f] = row[4 * mp + f];
+        }
+        if (mp >= 1 && L - mp <= G.N) {
+#pragma unroll
+          for (int f = 0; f < 4; f++) hi[q][f] = row[4 * (L - mp) + f];
+        }
+        twm[q] = to_c<C>(twg[mp]);
+      }
+    }
+    for (int i = tid; i < K; i += blockDim.x) tw[i] = to_c<C>(twg[i]);
+#pragma unroll
+    for (int q = 0; q < NIT; q++) {
+      const int it = tid + 256 * q, s = it / L, mp = it - s * L;
+      if (it < 2 * L) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+          C a = to_c<C>(lo[q][f]);
+          if (mp == 0) a.y = 0;
+          const C c = mp >= 1 && L - mp <= G.N ? cconj(to_c<C>(hi[q][f])) : mkc<C>(0, 0);
+          const C E = cadd(a, c), O = cmul(csub(a, c), cconj(twm[q]));
+          buf[(2 * f + s) * P + F::phys(mp)] = C{E.x - O.y, E.y + O.x};  // E + i O
+        }
+      }
+    }
   }
   __syncthreads();
   fft2<R1, R2, +1>(buf, 8, tw);
+  // mixed precision: the phi rows hold phi' = phi - phibar (phibar = phi_00 / sqrt 2, the area mean), so the phi
+  // fluxes and the Bernoulli function are formed with phi'; the forward transform's epilogue adds the mean's exact
+  // contribution -phibar delta (the mean's part of the Bernoulli function only reaches n = 0, whose tendency is 0)
   const T om = static_cast<T>(1 - mu[jn] * mu[jn]);  // identical for the southern row: mu(s) = -mu(n)
   const T g = static_cast<T>(G.g);
-  const int flags = mflags[b];
-  const bool untilted = flags & 1, flat = flags & 2;
   C res[10];
-  const int k = tid;  // one pair of longitudes (2k, 2k+1) per thread: L <= 256
   if (k < L) {
     const int pk = F::phys(F::pos(k));
 #pragma unroll
     for (int s = 0; s < 2; s++) {
       const int j = s ? js : jn;
       const C U = buf[(0 + s) * P + pk], V = buf[(2 + s) * P + pk], Z = buf[(4 + s) * P + pk], Ph = buf[(6 + s) * P + pk];
-      const size_t gi = (static_cast<size_t>(b) * G.J + j) * K + 2 * k;
       const T fj = static_cast<T>((2 * G.omega) * mu[j]);
       T o[5][2];
 #pragma unroll
       for (int e = 0; e < 2; e++) {
         const T u = e ? U.y : U.x, v = e ? V.y : V.x, z = e ? Z.y : Z.x, ph = e ? Ph.y : Ph.x;
         // untilted f = 2 Omega (-cos(lon) cos(lat) 0 + mu 1) = (2 Omega) mu exactly (sphere.cpp:116)
-        const T eta = z + (untilted ? fj : static_cast<T>(fgrid[gi + e]));
+        const T eta = z + (untilted ? fj : fv[s][e]);
         o[0][e] = u * eta;
         o[1][e] = v * eta;
         o[2][e] = u * ph;
         o[3][e] = v * ph;
-        o[4][e] = ph + g * (flat ? T(0) : static_cast<T>(hs[gi + e])) + (u * u + v * v) / (2 * om);  // g (h + hs) + E (sphere.cpp:357)
+        o[4][e] = ph + g * (flat ? T(0) : hsv[s][e]) + (u * u + v * v) / (2 * om);  // g (h + hs) + E (sphere.cpp:357)
       }
 #pragma unroll
       for (int q = 0; q < 5; q++) res[2 * q + s] = C{o[q][0], o[q][1]};
