@@ -381,18 +381,34 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   const int j = lane & 15, bl = mb + j;
   if (bl >= nb) return;
   const int b = b0 + bl, Km = G.N - m + 1;
+  // this lane's elements e = 4 ti + r: row 16 t + 4 (lane / 16) + r of the parity block, t = wave + NW ti
+  auto elem_k = [&](int e) { return 2 * (16 * (wave + NW * (e >> 2)) + 4 * (lane >> 4) + (e & 3)) + p; };
+  auto elem_ok = [&](int e) { return wave + NW * (e >> 2) < nrt && elem_k(e) < Km; };
+  if (mode == ANAL_TEND) {
+    // the RK4 update with the state loads of element e + 1 issued before the arithmetic and stores of element e
+    const double phibar = mixed ? xs[G.at(b, 2, 0)].x * 0.70710678118654752440 : 0.0;  // phi_00 P_0^0
+    TendIn next = {};
+    if (elem_ok(0)) next = tend_load(G, stage, b, static_cast<size_t>(G.moff[m]) + elem_k(0), x0, xs, acc, mixed);
 #pragma unroll
-  for (int ti = 0; ti < TPW; ti++) {
-    const int t = wave + NW * ti;
-    if (t >= nrt) break;
-    for (int r = 0; r < 4; r++) {
-      const int rl = 16 * t + 4 * (lane >> 4) + r;  // row within the parity block
-      const int k = 2 * rl + p;
-      if (k >= Km) continue;
+    for (int e = 0; e < 4 * TPW; e++) {
+      const TendIn cur = next;
+      if (e + 1 < 4 * TPW && elem_ok(e + 1))
+        next = tend_load(G, stage, b, static_cast<size_t>(G.moff[m]) + elem_k(e + 1), x0, xs, acc, mixed);
+      if (!elem_ok(e)) continue;
+      const int ti = e >> 2, r = e & 3;
       const double2 o1 = cx(accv[ti][0][r], accv[ti][1][r]), o2 = cx(accv[ti][2][r], accv[ti][3][r]);
       const double2 o3 = cx(accv[ti][4][r], accv[ti][5][r]), o4 = cx(accv[ti][6][r], accv[ti][7][r]);
-      anal_epilogue(G, mode, stage, dt, nu4, b, bl, m, k, o1, o2, o3, o4, x0, xs, acc, raw, mixed);
+      tend_store(G, stage, dt, nu4, b, m, elem_k(e), o1, o2, o3, o4, cur, phibar, x0, xs, acc, mixed);
     }
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < 4 * TPW; e++) {
+    if (!elem_ok(e)) continue;
+    const int ti = e >> 2, r = e & 3;
+    const double2 o1 = cx(accv[ti][0][r], accv[ti][1][r]), o2 = cx(accv[ti][2][r], accv[ti][3][r]);
+    const double2 o3 = cx(accv[ti][4][r], accv[ti][5][r]), o4 = cx(accv[ti][6][r], accv[ti][7][r]);
+    anal_epilogue(G, mode, stage, dt, nu4, b, bl, m, elem_k(e), o1, o2, o3, o4, x0, xs, acc, raw, mixed);
   }
 }
 
