@@ -305,36 +305,46 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   v4f accv[TPW][8];
   for (int t = 0; t < TPW; t++)
     for (int q = 0; q < 8; q++) accv[t][q] = v4f{0, 0, 0, 0};
-  for (int c0 = 0; c0 < KK; c0 += KC) {
-    __syncthreads();
-    // one thread per (member, pair of k): members fastest; a thread reads its two records of four contiguous
-    // complex operands and writes whole 32-bit LDS words
-    for (int e = threadIdx.x; e < (KC / 2) * MEM; e += blockDim.x) {
-      const int j = e % MEM, kk = 2 * (e / MEM);
-      const int bl = mb + j;
-      float vv[2][8];
+  // operand staging: one thread per (member, pair of k), members fastest; a thread reads its two records of four
+  // contiguous complex operands and writes whole 32-bit LDS words.  The next chunk's records are loaded into
+  // registers while the current chunk's MFMAs run.
+  constexpr int NIT = ((KC / 2) * MEM + 64 * NW - 1) / (64 * NW);
+  float vv[NIT][2][8];
+  auto stage_load = [&](int c0) {
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int k = c0 + kk + h;
         const int part = k >= G.Jhp ? 1 : 0, jh = part ? k - G.Jhp : k;
         float2 g[4] = {make_float2(0, 0), make_float2(0, 0), make_float2(0, 0), make_float2(0, 0)};
-        if (bl < nb && jh < G.Jh) {
+        if (e < (KC / 2) * MEM && bl < nb && jh < G.Jh) {
           const float2 *src = Gb + (((static_cast<size_t>(b0 + bl) * G.M1 + m) * 2 + p) * 2 + part) * 4 * G.Jh + static_cast<size_t>(jh) * 4;
 #pragma unroll
           for (int c = 0; c < 4; c++) g[c] = src[c];
         }
 #pragma unroll
-        for (int q = 0; q < 8; q++) vv[h][q] = (q & 1) ? g[q >> 1].y : g[q >> 1].x;
+        for (int q = 0; q < 8; q++) vv[it][h][q] = (q & 1) ? g[q >> 1].y : g[q >> 1].x;
       }
+    }
+  };
+  stage_load(0);
+  for (int c0 = 0; c0 < KK; c0 += KC) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+      const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM);
+      if (e >= (KC / 2) * MEM) continue;
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         if (PREC == LMW_FP32) {
-          bf[(q * KC + kk) * MEM + j] = vv[0][q];
-          bf[(q * KC + kk + 1) * MEM + j] = vv[1][q];
+          bf[(q * KC + kk) * MEM + j] = vv[it][0][q];
+          bf[(q * KC + kk + 1) * MEM + j] = vv[it][1][q];
         } else {
           __bf16 h0, l0, t0, h1, l1, t1;
-          split_bf16x3(vv[0][q], h0, l0, t0);
-          split_bf16x3(vv[1][q], h1, l1, t1);
+          split_bf16x3(vv[it][0][q], h0, l0, t0);
+          split_bf16x3(vv[it][1][q], h1, l1, t1);
           typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
           *reinterpret_cast<v2bf *>(bh + (q * MEM + j) * AP + kk) = v2bf{h0, h1};
           *reinterpret_cast<v2bf *>(blo + (q * MEM + j) * AP + kk) = v2bf{l0, l1};
@@ -342,6 +352,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
         }
       }
     }
+    if (c0 + KC < KK) stage_load(c0 + KC);
     __syncthreads();
 #pragma unroll
     for (int ti = 0; ti < TPW; ti++) {
