@@ -350,6 +350,94 @@ __device__ inline void march_iteration(March<T> &M, const Stencil<T> &S, const T
   }
 }
 
+// The interior rows of the march with the windows as rings instead of shifted arrays: row rho of the state, of the
+// RK4 sums and of f sits in slot rho mod 4, a stage input row in slot rho mod 2, the prefetched state row in slot
+// rho mod 2.  With PH = r mod 4 known at compile time every slot index is a constant, so four iterations unrolled
+// move no registers (the shifts took 12% of the FP32 interior loop's instructions).  The arithmetic is
+// march_iteration<true>'s.
+template <class T> struct Ring {
+  Row<T> x[4], n[2], acc[4], y[3][2];
+  T f[4];
+};
+template <int PH, int S, class T>
+__device__ inline void ring_stage(Ring<T> &R, StageCarry<T> &w, const Stencil<T> &S_, Row<T> &p, int r, int ny, int nx, int c0, int c1, bool owner,
+                                  T hdt, T fdt, T sdt, T *__restrict__ oh, T *__restrict__ ou, T *__restrict__ ov) {
+#pragma clang fp contract(off)
+  constexpr int XS = (PH - 1 - S) & 3;  // slot of row t = r - 1 - S
+  const int t = r - 1 - S;
+  Row<T> k;
+  const T f1 = R.f[(PH - S) & 3];
+  if constexpr (S == 0) march_row<true>(S_, w, R.x[(PH - 2) & 3], R.x[XS], p, t, ny, f1, k);
+  else {
+    march_row<true>(S_, w, R.y[S - 1][(PH - S) & 1], R.y[S - 1][(PH - 1 - S) & 1], p, t, ny, f1, k);
+    R.y[S - 1][(PH - S) & 1] = p;  // row t + 1 replaces row t - 1
+  }
+  const Row<T> &xs = R.x[XS];
+  Row<T> &a = R.acc[XS];
+  const T two = static_cast<T>(2);
+  if constexpr (S < 3) {
+    const T cs = S == 2 ? fdt : hdt;
+    if constexpr (S == 0) a = k;
+    else a = Row<T>{a.h + two * k.h, a.u + two * k.u, a.v + two * k.v};
+    p = Row<T>{xs.h + cs * k.h, xs.u + cs * k.u, xs.v + cs * k.v};
+  } else if (t >= c0 && t < c1 && owner) {
+    const size_t q = static_cast<size_t>(t) * nx;
+    oh[q] = xs.h + sdt * (a.h + k.h);
+    ou[q] = xs.u + sdt * (a.u + k.u);
+    ov[q] = xs.v + sdt * (a.v + k.v);
+  }
+}
+template <int PH, class T>
+__device__ inline void ring_iteration(Ring<T> &R, StageCarry<T> (&w)[4], const Stencil<T> &S, const T *__restrict__ fcor, int r, int ny, int nx,
+                                      int c0, int c1, bool owner, T hdt, T fdt, T sdt, const T *__restrict__ gh, const T *__restrict__ gu,
+                                      const T *__restrict__ gv, T *__restrict__ oh, T *__restrict__ ou, T *__restrict__ ov) {
+  const Row<T> fresh = R.n[PH & 1];
+  {
+    const size_t q = static_cast<size_t>(r + 1) * nx;  // r + 1 < ny in the interior loop
+    R.n[(PH + 1) & 1] = Row<T>{gh[q], gu[q], gv[q]};
+  }
+  Row<T> p = fresh;
+  ring_stage<PH, 0>(R, w[0], S, p, r, ny, nx, c0, c1, owner, hdt, fdt, sdt, oh, ou, ov);
+  ring_stage<PH, 1>(R, w[1], S, p, r, ny, nx, c0, c1, owner, hdt, fdt, sdt, oh, ou, ov);
+  ring_stage<PH, 2>(R, w[2], S, p, r, ny, nx, c0, c1, owner, hdt, fdt, sdt, oh, ou, ov);
+  ring_stage<PH, 3>(R, w[3], S, p, r, ny, nx, c0, c1, owner, hdt, fdt, sdt, oh, ou, ov);
+  R.x[PH & 3] = fresh;               // state row r replaces row r - 4
+  R.f[(PH + 1) & 3] = fcor[r + 1];  // f at corner row r + 1 replaces r - 3
+}
+// The windows between iterations, from march_iteration's shifted arrays to the rings (before iteration r, r = 0
+// mod 4) and back (before iteration r, r = 0 mod 4, after the last ring iteration).
+template <class T> __device__ inline void to_ring(const March<T> &M, Ring<T> &R) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) R.x[i] = M.x[i + 1];
+  R.n[0] = M.next;
+#pragma unroll
+  for (int i = 0; i < 3; i++) R.acc[i] = M.acc[i + 1];
+  R.f[0] = M.fnext;
+#pragma unroll
+  for (int i = 1; i < 4; i++) R.f[i] = M.f[i];
+#pragma unroll
+  for (int s = 1; s < 4; s++) {
+    R.y[s - 1][s & 1] = M.ym[s - 1];
+    R.y[s - 1][(s + 1) & 1] = M.yc[s - 1];
+  }
+}
+template <class T> __device__ inline void from_ring(const Ring<T> &R, March<T> &M) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) M.x[i + 1] = R.x[i];
+  M.next = R.n[0];
+  M.acc[0] = R.acc[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) M.acc[i + 1] = R.acc[i];
+#pragma unroll
+  for (int i = 1; i < 4; i++) M.f[i] = R.f[i];
+  M.fnext = R.f[0];
+#pragma unroll
+  for (int s = 1; s < 4; s++) {
+    M.ym[s - 1] = R.y[s - 1][s & 1];
+    M.yc[s - 1] = R.y[s - 1][(s + 1) & 1];
+  }
+}
+
 // One whole RK4 step (plane.cpp:143-158) by row march: a wavefront holds a strip of 64 columns (56 of them its own,
 // 4 on each side recomputed for the neighbours' stencils) and walks north through rows c0 - 4 .. c1 + 3 of one
 // member, loading each state row once.  Stage s works on row r - s while row r is loaded, so every stage's input at
@@ -384,9 +472,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES))) voi
     M.next = Row<T>{gh[q], gu[q], gv[q]};
     M.fnext = fcor[r0];
   }
-  const int lo = max(r0, 6), hi = min(r1 - 1, ny - 2);
+  // interior rows: every stage's row in 2..ny-2 and row r + 1 exists (FP32: from a multiple of 4, in groups of four)
+  const int lo = sizeof(T) == 4 && !GENERAL ? (max(r0, 6) + 3) & ~3 : max(r0, 6), hi = min(r1 - 1, ny - 2);
   int r = r0;
   for (; r <= r1 && r < lo; r++) march_iteration<false>(M, S, fcor, r, r1, ny, nx, c0, c1, owner, hdt, fdt, sdt, gh, gu, gv, oh, ou, ov);
+  // (FP32 only: in FP64 the unrolled rings need more than the 168 VGPRs of three wavefronts per SIMD, and two
+  // wavefronts per SIMD without the register moves ran no faster than three with them)
+  if constexpr (sizeof(T) == 4 && !GENERAL) {
+    if (r + 3 <= hi) {
+      Ring<T> R;
+      to_ring(M, R);
+      for (; r + 3 <= hi; r += 4) {
+        ring_iteration<0>(R, M.w, S, fcor, r, ny, nx, c0, c1, owner, hdt, fdt, sdt, gh, gu, gv, oh, ou, ov);
+        ring_iteration<1>(R, M.w, S, fcor, r + 1, ny, nx, c0, c1, owner, hdt, fdt, sdt, gh, gu, gv, oh, ou, ov);
+        ring_iteration<2>(R, M.w, S, fcor, r + 2, ny, nx, c0, c1, owner, hdt, fdt, sdt, gh, gu, gv, oh, ou, ov);
+        ring_iteration<3>(R, M.w, S, fcor, r + 3, ny, nx, c0, c1, owner, hdt, fdt, sdt, gh, gu, gv, oh, ou, ov);
+      }
+      from_ring(R, M);
+    }
+  }
   for (; r <= hi; r++) march_iteration<true>(M, S, fcor, r, r1, ny, nx, c0, c1, owner, hdt, fdt, sdt, gh, gu, gv, oh, ou, ov);
   for (; r <= r1; r++) march_iteration<false>(M, S, fcor, r, r1, ny, nx, c0, c1, owner, hdt, fdt, sdt, gh, gu, gv, oh, ou, ov);
 }
@@ -608,7 +712,7 @@ template <class T> struct PlaneCore : PlaneCoreBase {
   // in fcor.  tile 0 = rows per wavefront from the occupancy, 1..4 = 32, 64, 128, 256 rows, 5 = four stage launches
   int tile = 0;
   DevArray<T> fcor;
-  static constexpr int kMarchWaves = sizeof(T) == 8 ? 3 : 5;  // 168 / 96 VGPRs
+  static constexpr int kMarchWaves = sizeof(T) == 8 ? 3 : 4;  // 168 / 128 VGPRs: no spills in the interior loops
   int auto_rows = 0;
   int march_rows() {
     if (tile >= 1 && tile <= 4) return 32 << (tile - 1);
