@@ -252,7 +252,7 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
   const int j = lane & 15, bl = mb + j;
   if (bl >= nb) return;
   const int b = b0 + bl;
-  const double dm = m;
+  const double dm = m, ia = 1.0 / a;  // u, v are rounded to FP32 below: 1/a times instead of a division
   for (int r = 0; r < 4; r++) {
     const int jh = jt * 16 + 4 * (lane >> 4) + r;
     if (jh >= G.Jh) continue;
@@ -267,8 +267,8 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
       const double2 psH = cx(H(0), H(1)), chH = cx(H(2), H(3));
       const double2 u = csub(cim(dm, chP), psH), v = cadd(cim(dm, psP), chH);  // sphere.cpp:200-201
       float2 *o = Fs + ((static_cast<size_t>(b) * G.J + jj) * G.M1 + m) * 4;
-      o[0] = from_d2<float2>(cx(u.x / a, u.y / a));
-      o[1] = from_d2<float2>(cx(v.x / a, v.y / a));
+      o[0] = from_d2<float2>(cx(u.x * ia, u.y * ia));
+      o[1] = from_d2<float2>(cx(v.x * ia, v.y * ia));
       o[2] = from_d2<float2>(Z);
       o[3] = from_d2<float2>(F);
     }
