@@ -228,8 +228,9 @@ lmw_status lmw_ensemble_autotune(lmw_ensemble *e, int force, int *tiles, double 
 lmw_status lmw_sphere_autotune(lmw_sphere *s, int force, int *tiles, double *best_ms);
 /* set the four tile variant indices of group g directly (tests, or a caller's own tuning) */
 lmw_status lmw_ensemble_set_tiles(lmw_ensemble *e, int group, const int *tiles);
-/* the shape of tile variant `index`: kind 0 = FP64 members per block (a); 1 = mixed inverse row chunk (a);
- * 2 = mixed forward latitude chunk (a) and waves per block (b) */
+/* the shape of tile variant `index`: kind 0 = FP64 inverse members per block (a); 1 = mixed inverse row chunk (a);
+ * 2 = mixed forward latitude chunk (a) and waves per block (b); 3 = FP64 forward: members per block of the FMA
+ * kernel (a, b = 0) or the FP64 MFMA kernel with b waves per block (a = 0) */
 lmw_status lmw_tile_variants(int kind, int index, int *a, int *b);
 /* the same steps launched eagerly with a HIP event pair around every kernel, on the stream the kernels run on;
  * kernel_ms[3 g + 0..2] = total milliseconds of the inverse Legendre, grid-point (FFT + products) and forward

@@ -119,7 +119,8 @@ void autotune(SphereCore &c, bool force, int *tiles, double *best_ms) {
       c.tile_fp64_synth = t[0];
       c.launch_synth(c.xs.p, 0, c.B);
       c.launch_tend_grid(0, c.B, 0);  // valid analysis operands
-      for (int v = 0; v < kFp64Tiles; v++) {
+      c.build_fp64_mfma_tables();
+      for (int v = 0; v < kFp64AnalTiles; v++) {
         c.tile_fp64_anal = v;
         const double x = time_it(c, [&] { c.launch_anal(ANAL_TEND, 2, dt, 0, c.B, nullptr); });
         if (x < ba) ba = x, t[1] = v;
@@ -151,11 +152,13 @@ void autotune(SphereCore &c, bool force, int *tiles, double *best_ms) {
     append_cache(key, t);
   }
   for (int i = 0; i < 4; i++)
-    if (t[i] < 0 || t[i] >= (i < 2 ? kFp64Tiles : i == 2 ? kSynthTiles : kAnalTiles)) runtime("corrupt tile cache entry for " + key);
+    if (t[i] < 0 || t[i] >= (i == 0 ? kFp64Tiles : i == 1 ? kFp64AnalTiles : i == 2 ? kSynthTiles : kAnalTiles))
+      runtime("corrupt tile cache entry for " + key);
   c.tile_fp64_synth = t[0];
   c.tile_fp64_anal = t[1];
   c.tile_synth = t[2];
   c.tile_anal = t[3];
+  if (c.tile_fp64_anal >= kFp64Tiles) c.build_fp64_mfma_tables();  // before any graph capture
   c.drop_graph();
   if (tiles)
     for (int i = 0; i < 4; i++) tiles[i] = t[i];
@@ -191,10 +194,12 @@ lmw_status lmw_ensemble_set_tiles(lmw_ensemble *e, int group, const int *tiles) 
     std::vector<SphereCore *> gs = all_groups(e);
     if (group < 0 || group >= static_cast<int>(gs.size())) invalid("group index out of range");
     for (int i = 0; i < 4; i++)
-      if (tiles[i] < 0 || tiles[i] >= (i < 2 ? kFp64Tiles : i == 2 ? kSynthTiles : kAnalTiles)) invalid("tile index out of range");
+      if (tiles[i] < 0 || tiles[i] >= (i == 0 ? kFp64Tiles : i == 1 ? kFp64AnalTiles : i == 2 ? kSynthTiles : kAnalTiles))
+        invalid("tile index out of range");
     SphereCore &c = *gs[group];
     c.tile_fp64_synth = tiles[0];
     c.tile_fp64_anal = tiles[1];
+    if (c.tile_fp64_anal >= kFp64Tiles) c.build_fp64_mfma_tables();
     c.tile_synth = tiles[2];
     c.tile_anal = tiles[3];
     c.drop_graph();
@@ -219,6 +224,11 @@ lmw_status lmw_tile_variants(int kind, int index, int *a, int *b) {
         if (index < 0 || index >= kAnalTiles) invalid("tile index out of range");
         *a = kAnalKC[index];
         *b = kAnalNW[index];
+        break;
+      case 3:  // FP64 analysis: members per block of the FMA kernel (b = 0), or 0 and the waves of the MFMA kernel
+        if (index < 0 || index >= kFp64AnalTiles) invalid("tile index out of range");
+        *a = index < kFp64Tiles ? kFp64MB[index] : 0;
+        *b = index < kFp64Tiles ? 0 : index == kFp64Tiles ? 4 : 8;
         break;
       default: invalid("unknown tile kind");
     }

@@ -23,6 +23,7 @@
 namespace lmw {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 namespace {
@@ -43,6 +44,7 @@ constexpr int nsplit(int prec) { return prec == LMW_BF16X6 ? 3 : 2; }
 
 // acc += a * b for one 16x16 tile; PREC = LMW_FP32 consumes one float per lane, LMW_BF16X3 eight bf16 pairs
 __device__ inline v4f mma32(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+__device__ inline v4d mma64(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ inline v4f mma3(v8bf ah, v8bf al, v8bf bh, v8bf bl, v4f c) {
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
@@ -82,6 +84,16 @@ __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa,
     Abh[e] = x1;
     Abl[e] = x2;
     Ab3[e] = x3;
+  }
+}
+
+// The FP64 forward-transform A operand in A32's layout, [m][k = jh (P) | Jhp + jh (H)][r], one block per m.
+__global__ void k_build_anal64(Geo G, const double *__restrict__ Pa, const double *__restrict__ Ha, double *__restrict__ A64) {
+  const int m = blockIdx.x, kp = G.kpe[m] + G.kpo[m], KK = 2 * G.Jhp;
+  for (int e = threadIdx.x; e < KK * kp; e += blockDim.x) {
+    const int k = e / kp, r = e - k * kp, jh = k < G.Jhp ? k : k - G.Jhp;
+    const double *T = k < G.Jhp ? Pa : Ha;
+    A64[static_cast<size_t>(KK) * G.poff[m] + e] = T[static_cast<size_t>(G.poff[m]) * G.Jhp + static_cast<size_t>(jh) * kp + r];
   }
 }
 
@@ -280,7 +292,10 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
 struct AnalTabs {
   const float *A32;
   const __bf16 *Abh, *Abl, *Ab3;
+  const double *A64;  // FP64 copy of A32 (the FP64 MFMA variant)
 };
+// Operand storage of the forward transform: FP64 operands for the FP64 MFMA variant, FP32 for the mixed modes
+template <int PREC> using AnalIn = typename std::conditional<PREC == LMW_FP64, double2, float2>::type;
 
 // Block: one (m, parity), 16 members, 4 waves over the 16-row tiles of that parity.  The K dimension runs over the
 // northern latitudes twice (P with X, then H with Y), staged in LDS in chunks of 64.  Quantities q = o1 re/im ..
@@ -288,28 +303,34 @@ struct AnalTabs {
 // Tile shape (autotuned per truncation group): KC latitudes of K per LDS chunk, NW waves per block, each wave
 // owning row tiles wave, wave + NW, ... (TPW of them: 8 tiles of 16 rows cover the largest parity block, T170).
 // The work of one block (order m, parity p, members mb .. mb + 15), shared by the per-group and multi-group kernels.
+// PREC = LMW_FP64 runs the same schedule on FP64 operands with the FP64 MFMA (16x16x4): the FP64 step's forward
+// transform on matrix cores.
 template <int PREC, int KC, int NW>
-__device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, const float2 *__restrict__ Gb, int mode, int stage, double dt,
-                                              double nu4, double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
-                                              double2 *__restrict__ raw, int b0, int nb, int mixed, int m, int p, int mb) {
+__device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, const AnalIn<PREC> *__restrict__ Gb, int mode, int stage,
+                                              double dt, double nu4, double2 *__restrict__ x0, double2 *__restrict__ xs,
+                                              double2 *__restrict__ acc, double2 *__restrict__ raw, int b0, int nb, int mixed, int m, int p,
+                                              int mb) {
+  using ST = typename std::conditional<PREC == LMW_FP64, double, float>::type;  // staged operand type
+  using AV = typename std::conditional<PREC == LMW_FP64, v4d, v4f>::type;      // accumulator tile
+  using GT = AnalIn<PREC>;
   constexpr int AP = KC + 8;         // padded row pitch (bf16) of the BF16 chunk
   constexpr int TPW = (8 + NW - 1) / NW;
   extern __shared__ float lds_f[];
-  float *bf = lds_f;                                                         // FP32 chunk [q][k][member]
+  ST *bf = reinterpret_cast<ST *>(lds_f);                                    // FP32 / FP64 chunk [q][k][member]
   __bf16 *bh = reinterpret_cast<__bf16 *>(lds_f), *blo = bh + 8 * MEM * AP;  // BF16 chunk [q][member][k], parts 1 and 2
   __bf16 *b3p = blo + 8 * MEM * AP;                                           // part 3 (BF16X6)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Rp = p ? G.kpo[m] : kpe, rbase = p ? kpe : 0;
   if (Rp == 0) return;
   const int nrt = Rp / 16, KK = 2 * G.Jhp;
-  v4f accv[TPW][8];
+  AV accv[TPW][8];
   for (int t = 0; t < TPW; t++)
-    for (int q = 0; q < 8; q++) accv[t][q] = v4f{0, 0, 0, 0};
+    for (int q = 0; q < 8; q++) accv[t][q] = AV{0, 0, 0, 0};
   // operand staging: one thread per (member, pair of k), members fastest; a thread reads its two records of four
   // contiguous complex operands and writes whole 32-bit LDS words.  The next chunk's records are loaded into
   // registers while the current chunk's MFMAs run.
   constexpr int NIT = ((KC / 2) * MEM + 64 * NW - 1) / (64 * NW);
-  float vv[NIT][2][8];
+  ST vv[NIT][2][8];
   auto stage_load = [&](int c0) {
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
@@ -318,9 +339,9 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       for (int h = 0; h < 2; h++) {
         const int k = c0 + kk + h;
         const int part = k >= G.Jhp ? 1 : 0, jh = part ? k - G.Jhp : k;
-        float2 g[4] = {make_float2(0, 0), make_float2(0, 0), make_float2(0, 0), make_float2(0, 0)};
+        GT g[4] = {GT{0, 0}, GT{0, 0}, GT{0, 0}, GT{0, 0}};
         if (e < (KC / 2) * MEM && bl < nb && jh < G.Jh) {
-          const float2 *src = Gb + (((static_cast<size_t>(b0 + bl) * G.M1 + m) * 2 + p) * 2 + part) * 4 * G.Jh + static_cast<size_t>(jh) * 4;
+          const GT *src = Gb + (((static_cast<size_t>(b0 + bl) * G.M1 + m) * 2 + p) * 2 + part) * 4 * G.Jh + static_cast<size_t>(jh) * 4;
 #pragma unroll
           for (int c = 0; c < 4; c++) g[c] = src[c];
         }
@@ -338,7 +359,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       if (e >= (KC / 2) * MEM) continue;
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        if (PREC == LMW_FP32) {
+        if constexpr (PREC == LMW_FP32 || PREC == LMW_FP64) {
           bf[(q * KC + kk) * MEM + j] = vv[it][0][q];
           bf[(q * KC + kk + 1) * MEM + j] = vv[it][1][q];
         } else {
@@ -360,7 +381,14 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       if (t >= nrt) break;
       const int r = rbase + 16 * t + (lane & 15);
       const int kc = min(KC, KK - c0);  // KK is a multiple of 32
-      if (PREC == LMW_FP32) {
+      if constexpr (PREC == LMW_FP64) {
+        for (int s = 0; s < kc / 4; s++) {
+          const int kk = 4 * s + (lane >> 4);
+          const double av = T.A64[static_cast<size_t>(KK) * G.poff[m] + static_cast<size_t>(c0 + kk) * kp + r];
+#pragma unroll
+          for (int q = 0; q < 8; q++) accv[ti][q] = mma64(av, bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
+        }
+      } else if constexpr (PREC == LMW_FP32) {
         for (int s = 0; s < kc / 4; s++) {
           const int kk = 4 * s + (lane >> 4);
           const float av = T.A32[static_cast<size_t>(KK) * G.poff[m] + static_cast<size_t>(c0 + kk) * kp + r];
@@ -392,8 +420,12 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   const int j = lane & 15, bl = mb + j;
   if (bl >= nb) return;
   const int b = b0 + bl, Km = G.N - m + 1;
-  // this lane's elements e = 4 ti + r: row 16 t + 4 (lane / 16) + r of the parity block, t = wave + NW ti
-  auto elem_k = [&](int e) { return 2 * (16 * (wave + NW * (e >> 2)) + 4 * (lane >> 4) + (e & 3)) + p; };
+  // this lane's elements e = 4 ti + r: row 16 t + 4 (lane / 16) + r of the parity block, t = wave + NW ti (the
+  // FP64 MFMA interleaves: row 16 t + lane / 16 + 4 r)
+  auto elem_k = [&](int e) {
+    const int row = PREC == LMW_FP64 ? (lane >> 4) + 4 * (e & 3) : 4 * (lane >> 4) + (e & 3);
+    return 2 * (16 * (wave + NW * (e >> 2)) + row) + p;
+  };
   auto elem_ok = [&](int e) { return wave + NW * (e >> 2) < nrt && elem_k(e) < Km; };
   if (mode == ANAL_TEND) {
     // the RK4 update with the state loads of element e + 1 issued before the arithmetic and stores of element e
@@ -430,7 +462,7 @@ __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const doub
 }
 
 template <int PREC, int KC, int NW>
-__global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const float2 *__restrict__ Gb, int mode, int stage, double dt, double nu4,
+__global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const AnalIn<PREC> *__restrict__ Gb, int mode, int stage, double dt, double nu4,
                                                  double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
                                                  double2 *__restrict__ raw, int b0, int nb, int mixed) {
   anal_mx_block<PREC, KC, NW>(G, T, Gb, mode, stage, dt, nu4, x0, xs, acc, raw, b0, nb, mixed, blockIdx.x >> 1, blockIdx.x & 1, blockIdx.y * MEM);
@@ -563,12 +595,32 @@ void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
 
 template <int PREC, int KC, int NW>
 static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
-  AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p};
+  AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p, c.A64.p};
   dim3 grd(2 * c.M1, (nb + MEM - 1) / MEM), blk(64 * NW);
-  const size_t lds = PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float) : nsplit(PREC) * 8 * (KC + 8) * MEM * sizeof(__bf16);
-  hipLaunchKernelGGL((k_anal_mx<PREC, KC, NW>), grd, blk, lds, c.stream, c.geo(), T, c.Gb32.p, mode, stage, dt, c.prm.diffusion, c.x0.p, c.xs.p,
-                     c.acc.p, raw_out, b0, nb, 1);
+  const size_t lds = PREC == LMW_FP64   ? 8 * KC * MEM * sizeof(double)
+                     : PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float)
+                                        : nsplit(PREC) * 8 * (KC + 8) * MEM * sizeof(__bf16);
+  const AnalIn<PREC> *gb;
+  if constexpr (PREC == LMW_FP64) gb = c.Gb.p;
+  else gb = c.Gb32.p;
+  hipLaunchKernelGGL((k_anal_mx<PREC, KC, NW>), grd, blk, lds, c.stream, c.geo(), T, gb, mode, stage, dt, c.prm.diffusion, c.x0.p, c.xs.p,
+                     c.acc.p, raw_out, b0, nb, PREC == LMW_FP64 ? 0 : 1);
   LMW_LAUNCH_CHECK();
+}
+
+// The FP64 step's forward transform on the FP64 MFMA (FP64 analysis variants kFp64Tiles and kFp64Tiles + 1: KC = 32
+// and 4 or 8 waves).  Its table is built when such a variant is chosen, never inside a captured graph.
+void SphereCore::build_fp64_mfma_tables() {
+  if (A64.n) return;
+  A64.alloc(static_cast<size_t>(2 * Jhp) * R);
+  hipLaunchKernelGGL(k_build_anal64, dim3(M1), dim3(256), 0, stream, geo(), Pa.p, Ha.p, A64.p);
+  LMW_LAUNCH_CHECK();
+  sync();
+}
+void SphereCore::launch_anal_fp64_mfma(int nw, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  if (!A64.n) runtime("the FP64 MFMA tables were not built before the launch");
+  if (nw == 8) anal_mx_launch<LMW_FP64, 32, 8>(*this, mode, stage, dt, b0, nb, raw_out);
+  else anal_mx_launch<LMW_FP64, 32, 4>(*this, mode, stage, dt, b0, nb, raw_out);
 }
 
 template <int PREC>
@@ -670,7 +722,8 @@ static void anal_multi(const std::vector<SphereCore *> &gs, const std::vector<do
   MultiAnal M{};
   for (size_t g = 0; g < gs.size(); g++) {
     SphereCore &c = *gs[g];
-    M.g[g] = AnalGroup{c.geo(), AnalTabs{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p}, c.Gb32.p, c.x0.p, c.xs.p, c.acc.p, dts[g], c.prm.diffusion, c.B};
+    M.g[g] = AnalGroup{c.geo(), AnalTabs{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p, nullptr}, c.Gb32.p, c.x0.p, c.xs.p, c.acc.p, dts[g], c.prm.diffusion,
+                       c.B};
   }
   M.work = w.anal.p;
   M.nwork = w.nanal;

@@ -41,7 +41,9 @@ enum GridMode { GRID_DIAG = 0, GRID_RAW = 1 };
 
 // Tile variants of the Legendre kernels chosen per truncation group by the autotuner (autotune.hip): FP64
 // members per block; mixed-precision synthesis row chunk; mixed-precision analysis (latitude chunk, waves).
-constexpr int kFp64Tiles = 3, kSynthTiles = 2, kAnalTiles = 6;
+// tile variants: FP64 synthesis (members per block), FP64 analysis (members per block, then the FP64 MFMA kernel with
+// 4 or 8 waves), mixed synthesis (row chunk), mixed analysis (latitude chunk, waves)
+constexpr int kFp64Tiles = 3, kFp64AnalTiles = 5, kSynthTiles = 2, kAnalTiles = 6;
 constexpr int kMaxGroups = 4;  // truncation groups served by one multi-group (persistent) Legendre launch
 extern const int kFp64MB[kFp64Tiles];
 extern const int kSynthRC[kSynthTiles];
@@ -66,6 +68,7 @@ struct SphereCore {
   bool mixed_ready = false;
   DevArray<int> d_mrow, d_soff, d_ke32, d_ko32;
   DevArray<float> A32, Ps32, Hs32;
+  DevArray<double> A64;  // FP64 forward-transform operand of the FP64 MFMA variant (built on first use)
   DevArray<__bf16> Abh, Abl, Sph, Spl, Shh, Shl, Ab3, Sp3, Sh3;  // BF16 parts 1, 2 (and 3 for BF16X6)
   DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K
   DevArray<double2> x0, xs, acc;               // [b][3][S]
@@ -146,6 +149,8 @@ struct SphereCore {
   void launch_tracer_update(int stage, double dt);   // RK4 stage update of q from spec_io
   void launch_state_filter(double dt);               // del^4 filter of zeta, delta, phi (advect-only steps)
   void build_mixed_tables();
+  void build_fp64_mfma_tables();
+  void launch_anal_fp64_mfma(int nw, int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
   void launch_synth_mx(const double2 *spec, int b0, int nb);
   void launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
 

@@ -954,6 +954,10 @@ static void anal_launch(SphereCore &c, int mode, int stage, double dt, int b0, i
 }
 
 void SphereCore::launch_anal(int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
+  if (tile_fp64_anal >= kFp64Tiles) {  // the FP64 MFMA kernel
+    launch_anal_fp64_mfma(tile_fp64_anal == kFp64Tiles ? 4 : 8, mode, stage, dt, b0, nb, raw_out);
+    return;
+  }
   switch (kFp64MB[tile_fp64_anal]) {
     case 2: anal_launch<2>(*this, mode, stage, dt, b0, nb, raw_out); break;
     case 8: anal_launch<8>(*this, mode, stage, dt, b0, nb, raw_out); break;

@@ -432,8 +432,9 @@ def gauss_legendre(n: int):
 
 
 def tile_variant(kind: int, index: int):
-    """Shape of a tile variant: kind 0 FP64 members per block, 1 mixed inverse row chunk, 2 mixed forward
-    (latitude chunk, waves)."""
+    """Shape of a tile variant: kind 0 FP64 inverse members per block, 1 mixed inverse row chunk, 2 mixed forward
+    (latitude chunk, waves), 3 FP64 forward (members per block of the FMA kernel, 0) or (0, waves of the MFMA
+    kernel)."""
     a, b = C.c_int(), C.c_int()
     L.call("lmw_tile_variants", int(kind), int(index), C.byref(a), C.byref(b))
     return a.value, b.value
