@@ -111,7 +111,7 @@ void autotune(SphereCore &c, bool force, int *tiles, double *best_ms) {
     const double dt = 1.0;  // any dt: only the launch time matters, the state is restored
     if (!mx) {
       double bs = 1e30, ba = 1e30;
-      for (int v = 0; v < kFp64Tiles; v++) {
+      for (int v = 0; v < kFp64SynthTiles; v++) {
         c.tile_fp64_synth = v;
         const double x = time_it(c, [&] { c.launch_synth(c.xs.p, 0, c.B); });
         if (x < bs) bs = x, t[0] = v;
@@ -152,7 +152,7 @@ void autotune(SphereCore &c, bool force, int *tiles, double *best_ms) {
     append_cache(key, t);
   }
   for (int i = 0; i < 4; i++)
-    if (t[i] < 0 || t[i] >= (i == 0 ? kFp64Tiles : i == 1 ? kFp64AnalTiles : i == 2 ? kSynthTiles : kAnalTiles))
+    if (t[i] < 0 || t[i] >= (i == 0 ? kFp64SynthTiles : i == 1 ? kFp64AnalTiles : i == 2 ? kSynthTiles : kAnalTiles))
       runtime("corrupt tile cache entry for " + key);
   c.tile_fp64_synth = t[0];
   c.tile_fp64_anal = t[1];
@@ -194,7 +194,7 @@ lmw_status lmw_ensemble_set_tiles(lmw_ensemble *e, int group, const int *tiles) 
     std::vector<SphereCore *> gs = all_groups(e);
     if (group < 0 || group >= static_cast<int>(gs.size())) invalid("group index out of range");
     for (int i = 0; i < 4; i++)
-      if (tiles[i] < 0 || tiles[i] >= (i == 0 ? kFp64Tiles : i == 1 ? kFp64AnalTiles : i == 2 ? kSynthTiles : kAnalTiles))
+      if (tiles[i] < 0 || tiles[i] >= (i == 0 ? kFp64SynthTiles : i == 1 ? kFp64AnalTiles : i == 2 ? kSynthTiles : kAnalTiles))
         invalid("tile index out of range");
     SphereCore &c = *gs[group];
     c.tile_fp64_synth = tiles[0];
@@ -210,10 +210,10 @@ lmw_status lmw_tile_variants(int kind, int index, int *a, int *b) {
   return guard([&] {
     if (!a || !b) invalid("null pointer argument");
     switch (kind) {
-      case 0:
-        if (index < 0 || index >= kFp64Tiles) invalid("tile index out of range");
-        *a = kFp64MB[index];
-        *b = 0;
+      case 0:  // FP64 synthesis: members per block of the FMA kernel (b = 0), or 0 and the row chunk of the MFMA kernel
+        if (index < 0 || index >= kFp64SynthTiles) invalid("tile index out of range");
+        *a = index < kFp64Tiles ? kFp64MB[index] : 0;
+        *b = index < kFp64Tiles ? 0 : index == kFp64Tiles ? 32 : 64;
         break;
       case 1:
         if (index < 0 || index >= kSynthTiles) invalid("tile index out of range");

@@ -455,6 +455,96 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   }
 }
 
+// The FP64 inverse transform on the FP64 MFMA (16x16x4).  Block: one m, 8 members, one wave per 16 northern
+// latitudes.  The MFMA's 16 columns are the (member, re/im) pairs of one complex quantity (the P and H tables are
+// real), so a quantity takes one accumulator tile: P with zeta, phi, psi, chi and H with psi, chi, per parity
+// (sphere.cpp:150-162,179-206), folded into the north and south rows at the end.  Rows of one parity are staged in
+// LDS RC at a time as [quantity][row][2 member + re/im].  The FP64 MFMA holds row l/16 + 4r of its tile in lane l,
+// register r; the two components of a complex result sit in neighbouring lanes.
+template <int RC>
+__global__ __launch_bounds__(512) void k_synth_f64(Geo G, const double *__restrict__ Ps, const double *__restrict__ Hs,
+                                                   const double2 *__restrict__ spec, double2 *__restrict__ Fs, int b0, int nb) {
+  constexpr int MP = 8;
+  __shared__ double cs[4][RC][16];
+  const int m = blockIdx.x, mb = blockIdx.y * MP;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int Km = G.N - m + 1, kpe = G.kpe[m], kpo = G.kpo[m];
+  const double a = G.a;
+  const int jt = wave;
+  const bool active = jt * 16 < G.Jh;
+  v4d accP[2][4], accH[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) accP[p][q] = v4d{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 2; q++) accH[p][q] = v4d{0, 0, 0, 0};
+  }
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    const int kp = p ? kpo : kpe;  // rows of this parity (16-padded)
+    for (int c0 = 0; c0 < kp; c0 += RC) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < RC * MP; e += blockDim.x) {
+        const int j = e % MP, kk = e / MP, bl = mb + j, kq = c0 + kk, k = 2 * kq + p, n = m + k;
+        double2 z = cx(0, 0), d = cx(0, 0), f = cx(0, 0);
+        if (bl < nb && k < Km && kq < kp) {
+          const size_t q = static_cast<size_t>(G.moff[m]) + k;
+          z = spec[G.at(b0 + bl, 0, q)];
+          d = spec[G.at(b0 + bl, 1, q)];
+          f = spec[G.at(b0 + bl, 2, q)];
+        }
+        const double kn = n >= 1 ? -a * a / (n * (n + 1.0)) : 0.0;  // psi, chi = kn (zeta, delta): sphere.cpp:185-190
+        cs[0][kk][2 * j] = z.x;
+        cs[0][kk][2 * j + 1] = z.y;
+        cs[1][kk][2 * j] = f.x;
+        cs[1][kk][2 * j + 1] = f.y;
+        cs[2][kk][2 * j] = z.x * kn;
+        cs[2][kk][2 * j + 1] = z.y * kn;
+        cs[3][kk][2 * j] = d.x * kn;
+        cs[3][kk][2 * j + 1] = d.y * kn;
+      }
+      __syncthreads();
+      if (!active) continue;
+      const int jh = jt * 16 + (lane & 15), rbase = G.poff[m] + (p ? kpe : 0);
+      const int steps = min(RC, kp - c0) / 4;
+      for (int s = 0; s < steps; s++) {
+        const int kk = 4 * s + (lane >> 4);
+        const size_t ai = static_cast<size_t>(rbase + c0 + kk) * G.Jhp + jh;
+        const double aP = Ps[ai], aH = Hs[ai];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const double bq = cs[q][kk][lane & 15];
+          accP[p][q] = mma64(aP, bq, accP[p][q]);
+          if (q >= 2) accH[p][q - 2] = mma64(aH, bq, accH[p][q - 2]);
+        }
+      }
+    }
+  }
+  if (!active) return;
+  const int col = lane & 15, ri = col & 1, bl = mb + (col >> 1);
+  const double dm = m, sgn = ri ? 1.0 : -1.0;  // (i m x) has the component sgn m x' of the partner component x'
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int jh = jt * 16 + (lane >> 4) + 4 * r;
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      const double sg = s == 0 ? 1.0 : -1.0;
+      auto P = [&](int q) { return accP[0][q][r] + sg * accP[1][q][r]; };
+      auto H = [&](int q) { return sg * accH[0][q][r] + accH[1][q][r]; };
+      const double psPo = __shfl_xor(P(2), 1), chPo = __shfl_xor(P(3), 1);  // every lane takes part
+      const double u = sgn * dm * chPo - H(0), v = sgn * dm * psPo + H(1);  // sphere.cpp:200-201
+      if (jh >= G.Jh || bl >= nb) continue;
+      const int jj = s == 0 ? G.Jh + jh : G.Jh - 1 - jh;
+      double *o = reinterpret_cast<double *>(Fs + ((static_cast<size_t>(b0 + bl) * G.J + jj) * G.M1 + m) * 4) + ri;
+      o[0] = u / a;
+      o[2] = v / a;
+      o[4] = P(0);
+      o[6] = P(1);
+    }
+  }
+}
+
 template <int PREC, int RC>
 __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
                                                   int nb, int mixed) {
@@ -605,6 +695,21 @@ static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0
   else gb = c.Gb32.p;
   hipLaunchKernelGGL((k_anal_mx<PREC, KC, NW>), grd, blk, lds, c.stream, c.geo(), T, gb, mode, stage, dt, c.prm.diffusion, c.x0.p, c.xs.p,
                      c.acc.p, raw_out, b0, nb, PREC == LMW_FP64 ? 0 : 1);
+  LMW_LAUNCH_CHECK();
+}
+
+// The FP64 inverse transform on the FP64 MFMA (FP64 synthesis variants kFp64Tiles and kFp64Tiles + 1: RC = 32, 64)
+void SphereCore::launch_synth_fp64_mfma(int rc, const double2 *spec, int b0, int nb, bool scratch) {
+  Geo G = geo();
+  if (scratch) {  // member-major scratch buffers
+    G.sb = 3LL * S;
+    G.sf = S;
+    G.sq = 1;
+  }
+  const int waves = (Jh + 15) / 16;
+  dim3 grd(M1, (nb + 7) / 8), blk(64 * waves);
+  if (rc == 64) hipLaunchKernelGGL(k_synth_f64<64>, grd, blk, 0, stream, G, Ps.p, Hs.p, spec, Fs.p, b0, nb);
+  else hipLaunchKernelGGL(k_synth_f64<32>, grd, blk, 0, stream, G, Ps.p, Hs.p, spec, Fs.p, b0, nb);
   LMW_LAUNCH_CHECK();
 }
 

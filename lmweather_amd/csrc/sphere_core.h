@@ -41,9 +41,10 @@ enum GridMode { GRID_DIAG = 0, GRID_RAW = 1 };
 
 // Tile variants of the Legendre kernels chosen per truncation group by the autotuner (autotune.hip): FP64
 // members per block; mixed-precision synthesis row chunk; mixed-precision analysis (latitude chunk, waves).
-// tile variants: FP64 synthesis (members per block), FP64 analysis (members per block, then the FP64 MFMA kernel with
-// 4 or 8 waves), mixed synthesis (row chunk), mixed analysis (latitude chunk, waves)
-constexpr int kFp64Tiles = 3, kFp64AnalTiles = 5, kSynthTiles = 2, kAnalTiles = 6;
+// tile variants: FP64 synthesis (members per block, then the FP64 MFMA kernel with row chunks of 32 or 64), FP64
+// analysis (members per block, then the FP64 MFMA kernel with 4 or 8 waves), mixed synthesis (row chunk), mixed
+// analysis (latitude chunk, waves); the first kFp64Tiles FP64 variants are the FMA kernels' members per block
+constexpr int kFp64Tiles = 3, kFp64SynthTiles = 5, kFp64AnalTiles = 5, kSynthTiles = 2, kAnalTiles = 6;
 constexpr int kMaxGroups = 4;  // truncation groups served by one multi-group (persistent) Legendre launch
 extern const int kFp64MB[kFp64Tiles];
 extern const int kSynthRC[kSynthTiles];
@@ -151,6 +152,7 @@ struct SphereCore {
   void build_mixed_tables();
   void build_fp64_mfma_tables();
   void launch_anal_fp64_mfma(int nw, int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
+  void launch_synth_fp64_mfma(int rc, const double2 *spec, int b0, int nb, bool scratch);
   void launch_synth_mx(const double2 *spec, int b0, int nb);
   void launch_anal_mx(int mode, int stage, double dt, int b0, int nb, double2 *raw_out);
 

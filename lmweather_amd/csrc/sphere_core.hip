@@ -898,6 +898,10 @@ static void synth_launch(SphereCore &c, const double2 *spec, int b0, int nb, boo
 }
 
 void SphereCore::launch_synth(const double2 *spec, int b0, int nb, bool scratch) {
+  if (tile_fp64_synth >= kFp64Tiles) {  // the FP64 MFMA kernel
+    launch_synth_fp64_mfma(tile_fp64_synth == kFp64Tiles ? 32 : 64, spec, b0, nb, scratch);
+    return;
+  }
   switch (kFp64MB[tile_fp64_synth]) {
     case 2: synth_launch<2>(*this, spec, b0, nb, scratch); break;
     case 8: synth_launch<8>(*this, spec, b0, nb, scratch); break;

@@ -89,35 +89,40 @@ def test_every_tile_variant_matches_default(lmw):
             assert np.array_equal(x, y)
 
 
-def test_fp64_mfma_forward_transform(lmw):
-    """The FP64 forward transform on the FP64 MFMA (FP64 analysis variants 3 and 4: 4 and 8 waves) against the FMA
-    kernel: the same FP64 sums in another order, so the states agree to rounding after 4 steps, and so do the state
-    analyses of set_state_grid (the transform's STATE mode)."""
+@pytest.mark.parametrize("N", [85, 42])
+def test_fp64_mfma_transforms(lmw, N):
+    """The FP64 transforms on the FP64 MFMA (FP64 synthesis variants 3, 4: row chunks 32, 64; analysis variants 3, 4:
+    4 and 8 waves) against the FMA kernels: the same FP64 sums in another order, so the states agree to rounding after
+    4 steps, and so do the state analyses of set_state_grid (the forward transform's STATE mode) and the diagnostic
+    grids (the inverse transform outside the step).  9 members: a partial 8-member tile of the synthesis."""
     from lmweather_amd.initial import random_balanced_state
     from lmweather_amd.sphere import tile_variant
     assert tile_variant(3, 1) == (4, 0) and tile_variant(3, 3) == (0, 4) and tile_variant(3, 4) == (0, 8)
-    truncs = [85] * 6
+    assert tile_variant(0, 1) == (4, 0) and tile_variant(0, 3) == (0, 32) and tile_variant(0, 4) == (0, 64)
+    truncs = [N] * 9
     ref = build(lmw, truncs, 0)
     ref.steps([300.0], [4])
-    base = lmw.Sphere(85, lmw.SphereParams(*EARTH))
-    h, u, v = random_balanced_state(base, 40, 7)
+    base = lmw.Sphere(N, lmw.SphereParams(*EARTH))
+    h, u, v = random_balanced_state(base, N // 2, 7)
     g_ref = build(lmw, truncs, 0)
     g_ref.set_state_grid(2, h, u, v)
-    for var in (3, 4):
+    for sv, av in ((1, 3), (1, 4), (3, 1), (4, 1), (4, 3)):
         e = build(lmw, truncs, 0)
-        e.set_tiles(0, [1, var, 0, 1])
+        e.set_tiles(0, [sv, av, 0, 1])
         e.steps([300.0], [4])
         for x, y in zip(states(e), states(ref)):
-            assert rel_l2(x, y) <= 1e-13, var
+            assert rel_l2(x, y) <= 1e-13, (sv, av)
+        for f in range(3):
+            assert rel_l2(e.grid(8, f), ref.grid(8, f)) <= 1e-13, (sv, av, f)
         g = build(lmw, truncs, 0)
-        g.set_tiles(0, [1, var, 0, 1])
+        g.set_tiles(0, [sv, av, 0, 1])
         g.set_state_grid(2, h, u, v)
         zeta = np.linalg.norm(g_ref.get_spectral(2, 0))
         for f in range(3):
             a, b = g.get_spectral(2, f), g_ref.get_spectral(2, f)
             # delta of the balanced state is rounding noise (1e-17 of zeta): measured against zeta's size
             scale = np.linalg.norm(b) if f != 1 else zeta
-            assert np.linalg.norm(a - b) <= 1e-13 * scale, (var, f)
+            assert np.linalg.norm(a - b) <= 1e-13 * scale, (sv, av, f)
 
 
 def test_autotune_keeps_state_and_uses_cache(lmw, tmp_path, monkeypatch):
