@@ -545,17 +545,32 @@ __global__ __launch_bounds__(512) void k_synth_f64(Geo G, const double *__restri
   }
 }
 
+// XCD-aware block order of the per-group Legendre kernels: a 1-D grid of ceil(items / 8) * 8 * tiles blocks, the
+// member tiles of one item 8 block ids apart.  Blocks go to the 8 XCDs round robin and in id order, so the tiles of
+// an item run on one XCD at nearly the same time and the second reads the item's table rows from that XCD's L2.
+// Returns false for the padding blocks of the last group of 8.
+__device__ inline bool xcd_item(int items, int tiles, int &item, int &tile) {
+  const int id = blockIdx.x, group = 8 * tiles;
+  item = (id / group) * 8 + id % 8;
+  tile = (id / 8) % tiles;
+  return item < items;
+}
+
 template <int PREC, int RC>
 __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
-                                                  int nb, int mixed) {
-  synth_mx_block<PREC, RC>(G, T, spec, Fs, b0, nb, mixed, blockIdx.x, blockIdx.y * MEM);
+                                                  int nb, int mixed, int tiles) {
+  int m, t;
+  if (!xcd_item(G.M1, tiles, m, t)) return;
+  synth_mx_block<PREC, RC>(G, T, spec, Fs, b0, nb, mixed, m, t * MEM);
 }
 
 template <int PREC, int KC, int NW>
 __global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const AnalIn<PREC> *__restrict__ Gb, int mode, int stage, double dt, double nu4,
                                                  double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
-                                                 double2 *__restrict__ raw, int b0, int nb, int mixed) {
-  anal_mx_block<PREC, KC, NW>(G, T, Gb, mode, stage, dt, nu4, x0, xs, acc, raw, b0, nb, mixed, blockIdx.x >> 1, blockIdx.x & 1, blockIdx.y * MEM);
+                                                 double2 *__restrict__ raw, int b0, int nb, int mixed, int tiles) {
+  int mp, t;
+  if (!xcd_item(2 * G.M1, tiles, mp, t)) return;
+  anal_mx_block<PREC, KC, NW>(G, T, Gb, mode, stage, dt, nu4, x0, xs, acc, raw, b0, nb, mixed, mp >> 1, mp & 1, t * MEM);
 }
 
 // ------------------------------------------------------------------------------------------- multi-group kernels
@@ -663,9 +678,10 @@ template <int PREC, int RC>
 static void synth_mx_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
   SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Sp3.p, c.Sh3.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
   const int waves = (c.Jh + 15) / 16;
-  dim3 grd(c.M1, (nb + MEM - 1) / MEM), blk(64 * waves);
+  const int tiles = (nb + MEM - 1) / MEM;
+  dim3 grd((c.M1 + 7) / 8 * 8 * tiles), blk(64 * waves);
   const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : nsplit(PREC) * 8 * (RC + 8) * MEM * sizeof(__bf16);
-  hipLaunchKernelGGL((k_synth_mx<PREC, RC>), grd, blk, lds, c.stream, c.geo(), T, spec, c.Fs32.p, b0, nb, 1);
+  hipLaunchKernelGGL((k_synth_mx<PREC, RC>), grd, blk, lds, c.stream, c.geo(), T, spec, c.Fs32.p, b0, nb, 1, tiles);
   LMW_LAUNCH_CHECK();
 }
 
@@ -686,7 +702,8 @@ void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
 template <int PREC, int KC, int NW>
 static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
   AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p, c.A64.p};
-  dim3 grd(2 * c.M1, (nb + MEM - 1) / MEM), blk(64 * NW);
+  const int tiles = (nb + MEM - 1) / MEM;
+  dim3 grd((2 * c.M1 + 7) / 8 * 8 * tiles), blk(64 * NW);
   const size_t lds = PREC == LMW_FP64   ? 8 * KC * MEM * sizeof(double)
                      : PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float)
                                         : nsplit(PREC) * 8 * (KC + 8) * MEM * sizeof(__bf16);
@@ -694,7 +711,7 @@ static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0
   if constexpr (PREC == LMW_FP64) gb = c.Gb.p;
   else gb = c.Gb32.p;
   hipLaunchKernelGGL((k_anal_mx<PREC, KC, NW>), grd, blk, lds, c.stream, c.geo(), T, gb, mode, stage, dt, c.prm.diffusion, c.x0.p, c.xs.p,
-                     c.acc.p, raw_out, b0, nb, PREC == LMW_FP64 ? 0 : 1);
+                     c.acc.p, raw_out, b0, nb, PREC == LMW_FP64 ? 0 : 1, tiles);
   LMW_LAUNCH_CHECK();
 }
 
