@@ -47,6 +47,7 @@ typedef int lmw_status;
 #define LMW_GRID_V 2
 #define LMW_GRID_VORTICITY 3
 #define LMW_GRID_TRACER 4
+#define LMW_GRID_OROGRAPHY 5 /* the orography grid as set (grid_orography, cpp/sphere.hpp:50) */
 
 /* spectral fields (reference zeta_, delta_, phi_, q_: cpp/sphere.hpp:109) */
 #define LMW_SPEC_ZETA 0
@@ -176,6 +177,13 @@ lmw_status lmw_sphere_point(lmw_sphere *s, int which, double lat, double lon, do
 /* mode_coefficient(n, m, imag), wave_crest(m, lat): cpp/sphere.hpp:87-88 / sphere.cpp:639-652 */
 lmw_status lmw_sphere_mode_coefficient(lmw_sphere *s, int n, int m, int imag, double *out);
 lmw_status lmw_sphere_wave_crest(lmw_sphere *s, int m, double lat, double *out);
+/* text output, "%.10g" per number (the Sphere.save_csv / save_spectrum / save_zonal_mean operators,
+ * cpp/main.cpp:357-371, :328-335, :373-383): rows "lon lat h u v zeta q hs" in degrees with a blank line after each
+ * latitude row; "n ke pe" for n = 1..T; "lat mean_h mean_u" per latitude.  LMW_ERUNTIME "<who>: cannot write
+ * <path>" when the file cannot be written (main.cpp:86-92). */
+lmw_status lmw_sphere_save_csv(lmw_sphere *s, const char *path);
+lmw_status lmw_sphere_save_spectrum(lmw_sphere *s, const char *path);
+lmw_status lmw_sphere_save_zonal_mean(lmw_sphere *s, const char *path);
 /* the transforms exposed for tests: synthesize_scalar, analyze_scalar, analyze_divergence (cpp/sphere.hpp:97-100,
  * sphere.cpp:138-177) and winds_spectral_to_grid (sphere.cpp:179-206) */
 lmw_status lmw_sphere_synthesize(lmw_sphere *s, const double *spec, double *grid);
@@ -307,6 +315,21 @@ lmw_status lmw_plane_diag(lmw_plane *p, int member, int which, double *out);
 lmw_status lmw_plane_set_tile(lmw_plane *p, int tile);
 /* one evaluation of tendency(h, u, v) for one member: cpp/plane.cpp:59-129 */
 lmw_status lmw_plane_tendency(lmw_plane *p, int member, double *dh, double *du, double *dv);
+/* the parameters (Plane::params(), cpp/plane.hpp:24), members and precision of a plane; any pointer may be NULL */
+lmw_status lmw_plane_get_params(lmw_plane *p, lmw_plane_params *params, int *batch, int *precision);
+/* zonal harmonics of one member (cpp/plane.hpp:52-55 / plane.cpp:264-310): the amplitude 2|c_m| and a crest
+ * x in [0, lx/m) of harmonic m of h on row j, of v on v row j (1..ny-1), and the kinetic energy per unit mass in
+ * harmonic m averaged over the channel; LMW_EINVAL "zonal harmonic or row out of range" etc. as the reference */
+lmw_status lmw_plane_zonal_amplitude(lmw_plane *p, int member, int m, int j, double *out);
+lmw_status lmw_plane_zonal_crest(lmw_plane *p, int member, int m, int j, double *out);
+lmw_status lmw_plane_v_zonal_crest(lmw_plane *p, int member, int m, int j, double *out);
+lmw_status lmw_plane_harmonic_energy(lmw_plane *p, int member, int m, double *out);
+/* text output of one member, "%.10g" per number (the Plane.save_csv / save_row / save_column operators,
+ * cpp/main.cpp:464-493): rows "x y h u v zeta" at the cell centres with a blank line after each row (u, v averaged
+ * to the centre, zeta the mean of the four corners); "x h" along row `row`; "y h" along column `column` */
+lmw_status lmw_plane_save_csv(lmw_plane *p, int member, const char *path);
+lmw_status lmw_plane_save_row(lmw_plane *p, int member, int row, const char *path);
+lmw_status lmw_plane_save_column(lmw_plane *p, int member, int column, const char *path);
 
 #if defined(__GNUC__) || defined(__clang__)
 #pragma GCC visibility pop

@@ -134,6 +134,11 @@ class Sphere {
   std::vector<double> grid_u() const { return grid(LMW_GRID_U); }
   std::vector<double> grid_v() const { return grid(LMW_GRID_V); }
   std::vector<double> grid_vorticity() const { return grid(LMW_GRID_VORTICITY); }
+  std::vector<double> grid_orography() const { return grid(LMW_GRID_OROGRAPHY); }
+  // the text files of the Sphere.save_csv / save_spectrum / save_zonal_mean operators (cpp/main.cpp:328-383)
+  void save_csv(const std::string &path) const { check(lmw_sphere_save_csv(h_, path.c_str())); }
+  void save_spectrum(const std::string &path) const { check(lmw_sphere_save_spectrum(h_, path.c_str())); }
+  void save_zonal_mean(const std::string &path) const { check(lmw_sphere_save_zonal_mean(h_, path.c_str())); }
   std::vector<double> gauss_mu() const {
     std::vector<double> mu(nlat_), w(nlat_);
     check(lmw_sphere_gauss(h_, mu.data(), w.data()));
@@ -264,6 +269,28 @@ class Plane {
   double h_at(double x, double y) const { return interp(x, y, 0); }
   double u_at(double x, double y) const { return interp(x, y, 1); }
   double v_at(double x, double y) const { return interp(x, y, 2); }
+  // zonal harmonics of the device state (cpp/plane.cpp:264-310)
+  double zonal_amplitude(int m, int j) const { return harmonic(lmw_plane_zonal_amplitude, m, j); }
+  double zonal_crest(int m, int j) const { return harmonic(lmw_plane_zonal_crest, m, j); }
+  double v_zonal_crest(int m, int j) const { return harmonic(lmw_plane_v_zonal_crest, m, j); }
+  double harmonic_energy(int m) const {
+    double out;
+    check(lmw_plane_harmonic_energy(h_, 0, m, &out));
+    return out;
+  }
+  // cpp/plane.cpp:220-226, on the host copy
+  double zeta_at_corner(int i, int j) const {
+    const int nx = p_.nx, ny = p_.ny, im = (i + nx - 1) % nx;
+    auto U = [&](int a, int b) { return u_host_[static_cast<size_t>(b) * nx + a]; };
+    auto V = [&](int a, int b) { return v_host_[static_cast<size_t>(b) * nx + a]; };
+    const double dvdx = (V(i % nx, j) - V(im, j)) / dx();
+    const double uj = j < ny ? U(i % nx, j) : U(i % nx, ny - 1), ujm = j > 0 ? U(i % nx, j - 1) : U(i % nx, 0);
+    return dvdx - (uj - ujm) / dy();
+  }
+  // the text files of the Plane.save_csv / save_row / save_column operators (cpp/main.cpp:464-493)
+  void save_csv(const std::string &path) const { check(lmw_plane_save_csv(h_, 0, path.c_str())); }
+  void save_row(int row, const std::string &path) const { check(lmw_plane_save_row(h_, 0, row, path.c_str())); }
+  void save_column(int column, const std::string &path) const { check(lmw_plane_save_column(h_, 0, column, path.c_str())); }
   void set_h_from_grid(const std::vector<double> &g) { set(LMW_PLANE_H, g); }
   void set_u_from_grid(const std::vector<double> &g) { set(LMW_PLANE_U, g); }
   void set_v_from_grid(const std::vector<double> &g) { set(LMW_PLANE_V, g); }
@@ -278,6 +305,11 @@ class Plane {
   double diag(int which) const {
     double out;
     check(lmw_plane_diag(h_, 0, which, &out));
+    return out;
+  }
+  double harmonic(lmw_status (*fn)(lmw_plane *, int, int, int, double *), int m, int j) const {
+    double out;
+    check(fn(h_, 0, m, j, &out));
     return out;
   }
   void set(int field, const std::vector<double> &g) {

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblmw_hip.so")
 
 LMW_OK, LMW_EINVAL, LMW_ERUNTIME, LMW_EDEVICE = 0, 1, 2, 3
 FP64, FP32, BF16X3, BF16X6 = 0, 1, 2, 3
-GRID_H, GRID_U, GRID_V, GRID_VORTICITY, GRID_TRACER = 0, 1, 2, 3, 4
+GRID_H, GRID_U, GRID_V, GRID_VORTICITY, GRID_TRACER, GRID_OROGRAPHY = 0, 1, 2, 3, 4, 5
 SPEC_ZETA, SPEC_DELTA, SPEC_PHI, SPEC_Q = 0, 1, 2, 3
 (DIAG_MEAN_HEIGHT, DIAG_ENERGY, DIAG_KINETIC, DIAG_POTENTIAL, DIAG_ENSTROPHY, DIAG_MAX_WIND, DIAG_ERROR_L2, DIAG_ERROR_LINF,
  DIAG_TIME, DIAG_TRACER_MEAN, DIAG_TRACER_MASS, DIAG_TRACER_MIN, DIAG_TRACER_MAX, DIAG_TRACER_ERROR_L2,
@@ -142,6 +142,17 @@ SIGNATURES = {
     "lmw_plane_diag": [_vp, C.c_int, C.c_int, _dp],
     "lmw_plane_tendency": [_vp, C.c_int, _dp, _dp, _dp],
     "lmw_plane_set_tile": [_vp, C.c_int],
+    "lmw_plane_get_params": [_vp, C.POINTER(PlaneParams), _ip, _ip],
+    "lmw_plane_zonal_amplitude": [_vp, C.c_int, C.c_int, C.c_int, _dp],
+    "lmw_plane_zonal_crest": [_vp, C.c_int, C.c_int, C.c_int, _dp],
+    "lmw_plane_v_zonal_crest": [_vp, C.c_int, C.c_int, C.c_int, _dp],
+    "lmw_plane_harmonic_energy": [_vp, C.c_int, C.c_int, _dp],
+    "lmw_sphere_save_csv": [_vp, C.c_char_p],
+    "lmw_sphere_save_spectrum": [_vp, C.c_char_p],
+    "lmw_sphere_save_zonal_mean": [_vp, C.c_char_p],
+    "lmw_plane_save_csv": [_vp, C.c_int, C.c_char_p],
+    "lmw_plane_save_row": [_vp, C.c_int, C.c_int, C.c_char_p],
+    "lmw_plane_save_column": [_vp, C.c_int, C.c_int, C.c_char_p],
 }
 
 _lib = None

@@ -599,6 +599,37 @@ __global__ __launch_bounds__(256) void k_plane_rows_combine(int ny, const double
   if (t < 6) out[static_cast<size_t>(b) * 8 + t] = (t == 2 || t == 3) ? m : s.value();
 }
 
+// The zonal harmonic m of rows j0 .. j0 + gridDim.x - 1 of one field of member b (plane.cpp:264-268):
+// out[r] = (1/nx) sum_i row[i] exp(-2 pi i m (i + shift2 / 2) / nx).  The phase is reduced exactly in integers,
+// m (2 i + shift2) mod 2 nx, before sincospi, so no angle grows with m or i.  One block per row; each thread's
+// compensated sums, combined in thread order.
+template <class T>
+__global__ __launch_bounds__(256) void k_plane_harmonic(PlaneGeo P, const T *__restrict__ x, int b, int field, int m, int shift2, int j0,
+                                                        double *__restrict__ out) {
+  const int j = j0 + blockIdx.x, t = threadIdx.x, nx = P.nx, ny = P.ny;
+  const size_t nh = static_cast<size_t>(nx) * ny, nv = static_cast<size_t>(nx) * (ny + 1), per = 2 * nh + nv;
+  const T *row = x + b * per + (field == 0 ? 0 : field == 1 ? nh : 2 * nh) + static_cast<size_t>(j) * nx;
+  __shared__ double part[2][256];
+  Kahan re, im;
+  const int64_t period = 2LL * nx;
+  for (int i = t; i < nx; i += blockDim.x) {
+    const int64_t k = (static_cast<int64_t>(m) * (2 * i + shift2)) % period;
+    double sn, cs;
+    sincospi(static_cast<double>(k) / nx, &sn, &cs);
+    const double f = row[i];
+    re.add(f * cs);
+    im.add(-f * sn);
+  }
+  part[0][t] = re.value();
+  part[1][t] = im.value();
+  __syncthreads();
+  if (t < 2) {
+    Kahan s;
+    for (int q = 0; q < blockDim.x; q++) s.add(part[t][q]);
+    out[2 * static_cast<size_t>(blockIdx.x) + t] = s.value() / nx;
+  }
+}
+
 template <class T> __global__ void k_convert(const double *__restrict__ src, T *__restrict__ dst, size_t n) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = static_cast<T>(src[i]);
@@ -649,6 +680,8 @@ struct PlaneCoreBase {
   virtual void steps(double dt, int64_t count, double *kernel_ms = nullptr) = 0;
   virtual void reduce(double *out8B) = 0;  // B x 8
   virtual void tendency(int b, double *dh, double *du, double *dv) = 0;
+  // the zonal harmonic m of rows j0 .. j0 + rows - 1 of one field of member b, (re, im) per row into out[2 * rows]
+  virtual void harmonic(int b, int field, int m, int shift2, int j0, int rows, double *out) = 0;
   virtual void set_tile(int tile) = 0;
   virtual void reset() = 0;
   void sync() { LMW_HIP(hipStreamSynchronize(stream)); }
@@ -784,6 +817,15 @@ template <class T> struct PlaneCore : PlaneCoreBase {
     hipLaunchKernelGGL(k_plane_rows_combine, dim3(B), dim3(256), 0, stream, prm.ny, rowred.p, red.p);
     LMW_LAUNCH_CHECK();
     red.download(out, static_cast<size_t>(B) * 8, 0, stream);
+    sync();
+  }
+  DevArray<double> harm;
+  void harmonic(int b, int field, int m, int shift2, int j0, int rows, double *out) override {
+    if (rows < 1) return;
+    if (harm.n < 2 * static_cast<size_t>(rows)) harm.alloc(2 * static_cast<size_t>(rows));
+    hipLaunchKernelGGL(k_plane_harmonic<T>, dim3(rows), dim3(256), 0, stream, geo(), x0.p, b, field, m, shift2, j0, harm.p);
+    LMW_LAUNCH_CHECK();
+    harm.download(out, 2 * static_cast<size_t>(rows), 0, stream);
     sync();
   }
   void tendency(int b, double *dh, double *du, double *dv) override {
@@ -1046,6 +1088,81 @@ lmw_status lmw_plane_diag(lmw_plane *p, int member, int which, double *out) {
       case LMW_PDIAG_PV_ANOMALY_RMS: *out = std::sqrt(q[4] / (static_cast<double>(c.prm.ny - 1) * c.prm.nx)); break;
       default: invalid("unknown plane diagnostic");
     }
+  });
+}
+lmw_status lmw_plane_get_params(lmw_plane *p, lmw_plane_params *params, int *batch, int *precision) {
+  return guard([&] {
+    lmw_plane &pl = P(p);
+    if (params) *params = pl.core->prm;
+    if (batch) *batch = pl.core->B;
+    if (precision) *precision = pl.precision;
+  });
+}
+
+// Zonal harmonics (plane.cpp:264-310).  c_m of an h row is taken at the cell centres (shift 1/2), of a u row at
+// the west faces (shift 0), of a v row at the cell centres in x (shift 1/2).
+namespace {
+void check_harmonic(const lmw_plane &pl, int m, int j, int jmin, const char *what) {
+  if (m < 1 || m >= pl.core->prm.nx / 2 || j < jmin || j >= pl.core->prm.ny) invalid(what);
+}
+// the x in [0, lx/m) of a crest of 2|c| cos(2 pi m x / lx + arg c) (plane.cpp:276-284)
+double crest_x(const double *c, int m, double lx) {
+  constexpr double kPi = 3.14159265358979323846;
+  const double ph = std::atan2(c[1], c[0]);
+  double x = -ph / (2 * kPi * m) * lx;
+  const double per = lx / m;
+  x = std::fmod(x, per);
+  return x < 0 ? x + per : x;
+}
+}  // namespace
+lmw_status lmw_plane_zonal_amplitude(lmw_plane *p, int member, int m, int j, double *out) {
+  return guard([&] {
+    lmw_plane &pl = P(p);
+    check_member(pl, member);
+    NEEDP(out);
+    check_harmonic(pl, m, j, 0, "zonal harmonic or row out of range");
+    double c[2];
+    pl.core->harmonic(member, LMW_PLANE_H, m, 1, j, 1, c);
+    *out = 2 * std::hypot(c[0], c[1]);
+  });
+}
+lmw_status lmw_plane_zonal_crest(lmw_plane *p, int member, int m, int j, double *out) {
+  return guard([&] {
+    lmw_plane &pl = P(p);
+    check_member(pl, member);
+    NEEDP(out);
+    check_harmonic(pl, m, j, 0, "zonal harmonic or row out of range");
+    double c[2];
+    pl.core->harmonic(member, LMW_PLANE_H, m, 1, j, 1, c);
+    *out = crest_x(c, m, pl.core->prm.lx);
+  });
+}
+lmw_status lmw_plane_v_zonal_crest(lmw_plane *p, int member, int m, int j, double *out) {
+  return guard([&] {
+    lmw_plane &pl = P(p);
+    check_member(pl, member);
+    NEEDP(out);
+    check_harmonic(pl, m, j, 1, "zonal harmonic or v row out of range");
+    double c[2];
+    pl.core->harmonic(member, LMW_PLANE_V, m, 1, j, 1, c);
+    *out = crest_x(c, m, pl.core->prm.lx);
+  });
+}
+// sum over the u rows and the interior v rows of |c_m|^2, over ny (plane.cpp:286-299)
+lmw_status lmw_plane_harmonic_energy(lmw_plane *p, int member, int m, double *out) {
+  return guard([&] {
+    lmw_plane &pl = P(p);
+    check_member(pl, member);
+    NEEDP(out);
+    const int ny = pl.core->prm.ny;
+    if (m < 1 || m >= pl.core->prm.nx / 2) invalid("zonal harmonic out of range");
+    std::vector<double> cu(2 * static_cast<size_t>(ny)), cv(2 * static_cast<size_t>(ny - 1));
+    pl.core->harmonic(member, LMW_PLANE_U, m, 0, 0, ny, cu.data());
+    pl.core->harmonic(member, LMW_PLANE_V, m, 1, 1, ny - 1, cv.data());
+    double s = 0;
+    for (int j = 0; j < ny; j++) s += cu[2 * j] * cu[2 * j] + cu[2 * j + 1] * cu[2 * j + 1];
+    for (int j = 0; j < ny - 1; j++) s += cv[2 * j] * cv[2 * j] + cv[2 * j + 1] * cv[2 * j + 1];
+    *out = s / ny;
   });
 }
 lmw_status lmw_plane_tendency(lmw_plane *p, int member, double *dh, double *du, double *dv) {

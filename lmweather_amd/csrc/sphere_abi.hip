@@ -180,6 +180,12 @@ struct MemberView {
       c->tracer_grid(out);
       return;
     }
+    if (field == LMW_GRID_OROGRAPHY) {  // the stored hs grid (sphere.hpp:50)
+      const size_t plane = static_cast<size_t>(c->J) * c->K;
+      c->hs.download(out, plane, static_cast<size_t>(b) * plane, c->stream);
+      c->sync();
+      return;
+    }
     if (field < 0 || field > 3) invalid("unknown grid field");
     c->compute_grids(b, 1);
     c->download_grid(b, field, out);

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -132,6 +133,35 @@ class Plane:
 
     def potential_vorticity_anomaly_rms(self, member: int = 0) -> float:
         return self._diag(L.PDIAG_PV_ANOMALY_RMS, member)
+
+    # zonal harmonics (plane.cpp:264-310), on the device
+    def _harm(self, name, *args) -> float:
+        out = C.c_double()
+        L.call(name, self._h, *args, C.byref(out))
+        return out.value
+
+    def zonal_amplitude(self, m: int, j: int, member: int = 0) -> float:
+        return self._harm("lmw_plane_zonal_amplitude", member, m, j)
+
+    def zonal_crest(self, m: int, j: int, member: int = 0) -> float:
+        return self._harm("lmw_plane_zonal_crest", member, m, j)
+
+    def v_zonal_crest(self, m: int, j: int, member: int = 0) -> float:
+        return self._harm("lmw_plane_v_zonal_crest", member, m, j)
+
+    def harmonic_energy(self, m: int, member: int = 0) -> float:
+        return self._harm("lmw_plane_harmonic_energy", member, m)
+
+    # text output (main.cpp:464-493)
+    def save_csv(self, path: str, member: int = 0) -> None:
+        """Rows "x y h u v zeta" at the cell centres ("%.10g"), a blank line after each row."""
+        L.call("lmw_plane_save_csv", self._h, member, os.fsencode(path))
+
+    def save_row(self, row: int, path: str, member: int = 0) -> None:
+        L.call("lmw_plane_save_row", self._h, member, row, os.fsencode(path))
+
+    def save_column(self, column: int, path: str, member: int = 0) -> None:
+        L.call("lmw_plane_save_column", self._h, member, column, os.fsencode(path))
 
     def tendency(self, member: int = 0):
         dh, du = np.empty(self.nx * self.ny), np.empty(self.nx * self.ny)

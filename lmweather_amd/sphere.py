@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -243,6 +244,22 @@ class Sphere:
 
     def grid_vorticity(self) -> np.ndarray:
         return self._grid(L.GRID_VORTICITY)
+
+    def grid_orography(self) -> np.ndarray:  # sphere.hpp:50
+        return self._grid(L.GRID_OROGRAPHY)
+
+    # ---------------------------------------------------------------- text output (main.cpp:328-383)
+    def save_csv(self, path: str) -> None:
+        """Rows "lon lat h u v zeta q hs" (degrees, "%.10g"), a blank line after each latitude row."""
+        L.call("lmw_sphere_save_csv", self._h, os.fsencode(path))
+
+    def save_spectrum(self, path: str) -> None:
+        """Rows "n ke pe" for n = 1..T."""
+        L.call("lmw_sphere_save_spectrum", self._h, os.fsencode(path))
+
+    def save_zonal_mean(self, path: str) -> None:
+        """Rows "lat mean_h mean_u" per latitude."""
+        L.call("lmw_sphere_save_zonal_mean", self._h, os.fsencode(path))
 
     # ---------------------------------------------------------------- spectral state and transforms
     def get_spectral(self, field: int) -> np.ndarray:

@@ -1,6 +1,7 @@
 // The reference's native regression checks (reference tests/core.cpp:17-121) restated against the GPU library
 // through the C++ façade lmw.hpp: FFT against a naive DFT, Gauss-Legendre exactness, spectral round trip,
-// solid-body rotation, rest state and mass, the plane's linear gravity wave, and the parameter refusals.
+// solid-body rotation, rest state and mass, the plane's linear gravity wave and its zonal harmonic, and the parameter
+// refusals.
 // Same tolerances as the reference.  Built by __graft_entry__.build() into tests/bin/core_gpu; run by
 // tests/test_gpu_cpp.py.
 #include <cmath>
@@ -120,9 +121,17 @@ int main() {
     for (int j = 0; j < q.ny; j++)
       for (int i = 0; i < q.nx; i++) pl.h(i, j) = q.depth + std::cos(k * pl.x_center(i));
     pl.push();
+    CLOSE(pl.zonal_amplitude(1, 3), 1.0, 1e-12);
+    const double x0 = pl.zonal_crest(1, 3);  // 0, or just below lx
+    CLOSE(std::fmin(x0, q.lx - x0), 0.0, 1e-6);
+    CLOSE(pl.harmonic_energy(1), 0.0, 0.0);
+    CLOSE(pl.zeta_at_corner(5, 3), 0.0, 0.0);
     pl.run(pi / (c * k), 0);
     CLOSE(pl.h_at(0.5 * pl.dx(), 1e4) - q.depth, -std::cos(k * 0.5 * pl.dx()), 2e-3);
     CLOSE(pl.mean_depth(), q.depth, 1e-10);
+    // half a period later the crest sits at lx / 2 with the same amplitude (tests/plane_gravity_wave.lms:15)
+    CLOSE(pl.zonal_amplitude(1, 3), 1.0, 5e-4);
+    CLOSE(pl.zonal_crest(1, 3), q.lx / 2, 2500.0);
     bool threw = false;
     try {
       lmw::PlaneParams bad;
