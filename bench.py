@@ -284,9 +284,63 @@ def ensemble_bench(lmw, args) -> dict:
     return out
 
 
+def setup_bench(lmw, args):
+    """The ensemble initial-condition path: set_state_grid of `members` T170 members from host grids, one member per
+    call (the reference's shape) against the batched call (lmw_ensemble_set_state_grids), and the device-side
+    Williamson 6 set-up (lmw_ensemble_setup: no host grids).  value = member set-ups/s of the batched call; the CPU
+    baseline is the reference's own set_state_grid (oracle/_ref, else the oracle) on one core."""
+    members = args.members if args.members != 32 else 256
+    prec = PREC_NAMES.index(args.precision)
+    h, u, v, hs = initial_state(lmw)
+    H, U, V = (np.ascontiguousarray(np.broadcast_to(x, (members,) + x.shape)) for x in (h, u, v))
+    ens = lmw.Ensemble([N] * members, lmw.SphereParams(A, OM, G), prec)
+    ens.set_state_grid(0, h, u, v)  # warm-up: first launches, buffers
+    ens.set_state_grids(0, 2, H[:2], U[:2], V[:2])
+    ens.setup(0, 2, lmw._lib.SETUP_WILLIAMSON6)
+    t0 = time.perf_counter()
+    for i in range(members):
+        ens.set_state_grid(i, h, u, v)
+    t_one = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.set_state_grids(0, members, H, U, V)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.setup(0, members, lmw._lib.SETUP_WILLIAMSON6)
+    t_dev = time.perf_counter() - t0
+    out = {
+        "metric": "ensemble set-up: member set_state_grid/s at T170",
+        "value": members / t_batch,
+        "unit": "member set-ups/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * t_batch,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{members} T170 members (512x256 Gaussian grid): set_state_grid from host grids of the "
+                               "balanced random flow; one step = the whole ensemble set up once", "members": members,
+                   "precision": args.precision},
+        "one_member_per_call_per_s": members / t_one,
+        "device_williamson6_per_s": members / t_dev,
+        "batched_speedup": t_one / t_batch,
+        "host_grid_bytes": int(H.nbytes * 3),
+    }
+    del ens
+    if not args.no_cpu:
+        from oracle import pyapi
+        kind = "reference" if pyapi.available("ref") else "port"
+        s = pyapi.OracleSphere(N, A, OM, G, kind="ref" if kind == "reference" else "oracle")
+        s.set_state_grid(h, u, v)
+        n = 10
+        t0 = time.perf_counter()
+        for _ in range(n):
+            s.set_state_grid(h, u, v)
+        wall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / wall, "unit": "member set-ups/s", "cores": 1, "kind": kind,
+                               "sample": f"{n} set_state_grid calls of one T170 member, single thread, {wall:.2f} s"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble"])
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=4)
@@ -310,6 +364,9 @@ def main() -> int:
         return 0
     if args.workload == "ensemble":
         print(json.dumps(ensemble_bench(lmw, args)))
+        return 0
+    if args.workload == "setup":
+        print(json.dumps(setup_bench(lmw, args)))
         return 0
     prec = PREC_NAMES.index(args.precision)
     mixed = prec != lmw.FP64

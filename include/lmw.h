@@ -73,6 +73,13 @@ typedef int lmw_status;
 #define LMW_DIAG_TRACER_ERROR_L2 13
 #define LMW_DIAG_TRACER_ERROR_LINF 14
 
+/* analytic initial states of lmw_sphere_setup / lmw_ensemble_setup: the test cases of Williamson et al. (1992,
+ * J. Comput. Phys. 102:211-224) behind the Sphere.williamson1/2/5/6 operators (reference cpp/main.cpp:174-291) */
+#define LMW_SETUP_WILLIAMSON1 1 /* cosine bell tracer in solid-body rotation about an axis tilted by alpha; advect-only */
+#define LMW_SETUP_WILLIAMSON2 2 /* steady geostrophic solid-body rotation about an axis tilted by alpha */
+#define LMW_SETUP_WILLIAMSON5 5 /* zonal flow over a conical mountain (sets the orography) */
+#define LMW_SETUP_WILLIAMSON6 6 /* Rossby-Haurwitz wave of wavenumber 4 */
+
 /* point evaluators of lmw_sphere_point (reference height_at/wind_at/vorticity_at: cpp/sphere.hpp:83-86) */
 #define LMW_POINT_HEIGHT 0
 #define LMW_POINT_U 1
@@ -177,6 +184,11 @@ lmw_status lmw_sphere_point(lmw_sphere *s, int which, double lat, double lon, do
 /* mode_coefficient(n, m, imag), wave_crest(m, lat): cpp/sphere.hpp:87-88 / sphere.cpp:639-652 */
 lmw_status lmw_sphere_mode_coefficient(lmw_sphere *s, int n, int m, int imag, double *out);
 lmw_status lmw_sphere_wave_crest(lmw_sphere *s, int m, double lat, double *out);
+/* set-up: one of the LMW_SETUP_* states, evaluated at the Gauss grid and analysed on the device (no host grids);
+ * williamson1(alpha) / williamson2(alpha) set the axis tilt to alpha, williamson5() / williamson6() to 0, as the
+ * reference's operators do (cpp/main.cpp:174-291).  LMW_EINVAL "angle must be a finite number" for a non-finite
+ * alpha. */
+lmw_status lmw_sphere_setup(lmw_sphere *s, int which, double alpha);
 /* text output, "%.10g" per number (the Sphere.save_csv / save_spectrum / save_zonal_mean operators,
  * cpp/main.cpp:357-371, :328-335, :373-383): rows "lon lat h u v zeta q hs" in degrees with a blank line after each
  * latitude row; "n ke pe" for n = 1..T; "lat mean_h mean_u" per latitude.  LMW_ERUNTIME "<who>: cannot write
@@ -205,6 +217,12 @@ lmw_status lmw_ensemble_create(int members, const int *truncations, const lmw_sp
 lmw_status lmw_ensemble_destroy(lmw_ensemble *e);
 lmw_status lmw_ensemble_member_dims(const lmw_ensemble *e, int member, int *truncation, int *nlat, int *nlon);
 lmw_status lmw_ensemble_set_state_grid(lmw_ensemble *e, int member, const double *h, const double *u, const double *v, size_t n);
+/* set_state_grid of members member0 .. member0 + count - 1 (one truncation) from grids [count][nlat][nlon] each, in
+ * batched launches (no reference counterpart: the ensemble set-up path); n = count * nlat * nlon */
+lmw_status lmw_ensemble_set_state_grids(lmw_ensemble *e, int member0, int count, const double *h, const double *u, const double *v, size_t n);
+/* lmw_sphere_setup for members member0 .. member0 + count - 1 in batched launches; LMW_SETUP_WILLIAMSON1 needs a
+ * tracer, which ensemble members do not carry (LMW_EINVAL) */
+lmw_status lmw_ensemble_setup(lmw_ensemble *e, int member0, int count, int which, double alpha);
 lmw_status lmw_ensemble_set_orography_grid(lmw_ensemble *e, int member, const double *hs, size_t n);
 lmw_status lmw_ensemble_set_axis_tilt(lmw_ensemble *e, int member, double alpha);
 lmw_status lmw_ensemble_set_spectral(lmw_ensemble *e, int member, int field, const double *re_im, size_t n_complex);

@@ -135,6 +135,8 @@ class Sphere {
   std::vector<double> grid_v() const { return grid(LMW_GRID_V); }
   std::vector<double> grid_vorticity() const { return grid(LMW_GRID_VORTICITY); }
   std::vector<double> grid_orography() const { return grid(LMW_GRID_OROGRAPHY); }
+  // a Williamson et al. (1992) test case set up on the device (LMW_SETUP_*; the williamson1/2/5/6 operators)
+  void setup(int which, double alpha = 0.0) { check(lmw_sphere_setup(h_, which, alpha)); }
   // the text files of the Sphere.save_csv / save_spectrum / save_zonal_mean operators (cpp/main.cpp:328-383)
   void save_csv(const std::string &path) const { check(lmw_sphere_save_csv(h_, path.c_str())); }
   void save_spectrum(const std::string &path) const { check(lmw_sphere_save_spectrum(h_, path.c_str())); }

@@ -20,6 +20,7 @@ SPEC_ZETA, SPEC_DELTA, SPEC_PHI, SPEC_Q = 0, 1, 2, 3
  DIAG_TIME, DIAG_TRACER_MEAN, DIAG_TRACER_MASS, DIAG_TRACER_MIN, DIAG_TRACER_MAX, DIAG_TRACER_ERROR_L2,
  DIAG_TRACER_ERROR_LINF) = range(15)
 POINT_HEIGHT, POINT_U, POINT_V, POINT_VORTICITY, POINT_TRACER = 0, 1, 2, 3, 4
+SETUP_WILLIAMSON1, SETUP_WILLIAMSON2, SETUP_WILLIAMSON5, SETUP_WILLIAMSON6 = 1, 2, 5, 6
 PLANE_H, PLANE_U, PLANE_V = 0, 1, 2
 PDIAG_MEAN_DEPTH, PDIAG_ENERGY, PDIAG_MAX_SPEED, PDIAG_PV_ANOMALY_RMS, PDIAG_TIME = range(5)
 
@@ -147,6 +148,9 @@ SIGNATURES = {
     "lmw_plane_zonal_crest": [_vp, C.c_int, C.c_int, C.c_int, _dp],
     "lmw_plane_v_zonal_crest": [_vp, C.c_int, C.c_int, C.c_int, _dp],
     "lmw_plane_harmonic_energy": [_vp, C.c_int, C.c_int, _dp],
+    "lmw_sphere_setup": [_vp, C.c_int, C.c_double],
+    "lmw_ensemble_setup": [_vp, C.c_int, C.c_int, C.c_int, C.c_double],
+    "lmw_ensemble_set_state_grids": [_vp, C.c_int, C.c_int, _dp, _dp, _dp, C.c_size_t],
     "lmw_sphere_save_csv": [_vp, C.c_char_p],
     "lmw_sphere_save_spectrum": [_vp, C.c_char_p],
     "lmw_sphere_save_zonal_mean": [_vp, C.c_char_p],

@@ -480,6 +480,12 @@ lmw_status lmw_sphere_add_mode(lmw_sphere *s, int n, int m, double amplitude, do
     s->view().add_mode(n, m, amplitude, mean_depth);
   });
 }
+lmw_status lmw_sphere_setup(lmw_sphere *s, int which, double alpha) {
+  return guard([&] {
+    NEED(s);
+    s->core->setup(0, 1, which, alpha);
+  });
+}
 lmw_status lmw_sphere_perturb(lmw_sphere *s, double amplitude, uint64_t seed, int max_wave) {
   return guard([&] {
     NEED(s);
@@ -736,6 +742,49 @@ lmw_status lmw_ensemble_set_state_grid(lmw_ensemble *e, int member, const double
     NEED(h && u && v);
     mv.c->set_state_grid(mv.b, h, u, v);
     mv.c->sync();
+  });
+}
+}  // extern "C"
+
+// The members [member0, member0 + count) as runs that sit in one group with consecutive local indices:
+// fn(core, first local index, run length, offset of the run in the range) once per run.  same_truncation: refuse a
+// range whose members differ in truncation (the host grids of one call share a size).
+template <class F> static void member_runs(lmw_ensemble *e, int member0, int count, bool same_truncation, F &&fn) {
+  if (count < 0) invalid("member count must not be negative");
+  if (count == 0) return;
+  if (member0 < 0 || member0 + static_cast<int64_t>(count) > static_cast<int64_t>(e->where.size())) invalid("member index out of range");
+  const int N = e->view(member0).c->N;
+  int i = member0;
+  while (i < member0 + count) {
+    MemberView v = e->view(i);
+    if (same_truncation && v.c->N != N) invalid("the members of one call must share a truncation");
+    int n = 1;
+    while (i + n < member0 + count && e->where[i + n].first == e->where[i].first && e->where[i + n].second == v.b + n) n++;
+    fn(*v.c, v.b, n, i - member0);
+    i += n;
+  }
+}
+
+extern "C" {
+lmw_status lmw_ensemble_set_state_grids(lmw_ensemble *e, int member0, int count, const double *h, const double *u, const double *v, size_t n) {
+  return guard([&] {
+    NEED(e);
+    if (count > 0) {
+      MemberView mv = e->view(member0);
+      if (n != static_cast<size_t>(count) * mv.ngrid()) invalid("grid size mismatch");
+      NEED(h && u && v);
+    }
+    member_runs(e, member0, count, true, [&](SphereCore &c, int b0, int nb, int off) {
+      const size_t o = static_cast<size_t>(off) * c.J * c.K;
+      c.set_state_grids(b0, nb, h + o, u + o, v + o);
+    });
+  });
+}
+lmw_status lmw_ensemble_setup(lmw_ensemble *e, int member0, int count, int which, double alpha) {
+  return guard([&] {
+    NEED(e);
+    if (which == LMW_SETUP_WILLIAMSON1) invalid("a tracer needs a single-member sphere");
+    member_runs(e, member0, count, false, [&](SphereCore &c, int b0, int nb, int) { c.setup(b0, nb, which, alpha); });
   });
 }
 lmw_status lmw_ensemble_set_orography_grid(lmw_ensemble *e, int member, const double *hs, size_t n) {

@@ -45,6 +45,8 @@ enum GridMode { GRID_DIAG = 0, GRID_RAW = 1 };
 // analysis (members per block, then the FP64 MFMA kernel with 4 or 8 waves), mixed synthesis (row chunk), mixed
 // analysis (latitude chunk, waves); the first kFp64Tiles FP64 variants are the FMA kernels' members per block
 constexpr int kFp64Tiles = 3, kFp64SynthTiles = 5, kFp64AnalTiles = 5, kSynthTiles = 2, kAnalTiles = 6;
+constexpr int kSetupChunk = 32;  // members per launch of the batched set-up path (bounds the staging buffer in3)
+constexpr int SETUP_TRACER = 100;  // k_setup: test case 1's tracer bell (internal; not an LMW_SETUP_* value)
 constexpr int kMaxGroups = 4;  // truncation groups served by one multi-group (persistent) Legendre launch
 extern const int kFp64MB[kFp64Tiles];
 extern const int kSynthRC[kSynthTiles];
@@ -107,6 +109,12 @@ struct SphereCore {
   void update_flags(int b);
   void set_orography(int b, const double *hs_host);
   void set_state_grid(int b, const double *h, const double *u, const double *v);
+  void set_state_grids(int b0, int nb, const double *h, const double *u, const double *v);
+  void setup(int b0, int nb, int which, double alpha);
+  void ensure_in3(int nb);
+  void state_from_in3(int b0, int nb);
+  void mark_reference_range(int b0, int nb);
+  void tracer_from_in3();
   void set_spectral_packed(int b, int field, const double2 *packed);  // host packed triangle
   void get_spectral_packed(int b, int field, double2 *packed);
   void mark_reference(int b);

@@ -163,6 +163,60 @@ __global__ void k_coriolis(Geo G, const double *mu, double *f, double alpha) {
   }
 }
 
+// Analytic initial states of Williamson et al. (1992, J. Comput. Phys. 102:211-224) at the Gauss grid: the set-up
+// operators of cpp/main.cpp:174-291.  in3[local member][h, u, v][j][k] for the members of blockIdx.y; test case 5 also
+// writes its conical mountain into hs[member], and SETUP_TRACER writes test case 1's cosine bell into plane 0 of in3.
+// lat = asin(mu_j), lon = 2 pi k / K (sphere.cpp:120-121).
+__global__ void k_setup(Geo G, int which, double alpha, const double *__restrict__ mu, double *__restrict__ in3,
+                        double *__restrict__ hs, int b0) {
+  const int j = blockIdx.x, bl = blockIdx.y, K = G.K;
+  const size_t plane = static_cast<size_t>(G.J) * K, row = static_cast<size_t>(j) * K;
+  const double pi = 3.14159265358979323846, a = G.a, om = G.omega, g = G.g;
+  const double lat = asin(mu[j]), cl = cos(lat), sl = sin(lat);
+  double *h = in3 + static_cast<size_t>(bl) * 3 * plane + row, *u = h + plane, *v = u + plane;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    const double lon = 2 * pi * k / K;
+    if (which == LMW_SETUP_WILLIAMSON2 || which == LMW_SETUP_WILLIAMSON1 || which == SETUP_TRACER) {
+      // solid-body rotation about an axis tilted by alpha, once round in 12 days (test cases 1 and 2)
+      const double u0 = 2 * pi * a / (12 * 86400.0);
+      if (which == SETUP_TRACER) {  // cosine bell of 1000 m, radius a/3, centred at (270 E, 0 N)
+        const double R = a / 3, lonc = 1.5 * pi;
+        const double c = cos(lat) * cos(lon - lonc);  // sin(latc) = 0
+        const double rr = a * acos(fmax(-1.0, fmin(1.0, c)));
+        h[k] = rr < R ? 500.0 * (1 + cos(pi * rr / R)) : 0.0;
+        continue;
+      }
+      u[k] = u0 * (cl * cos(alpha) + sl * cos(lon) * sin(alpha));
+      v[k] = -u0 * sin(lon) * sin(alpha);
+      if (which == LMW_SETUP_WILLIAMSON1) {
+        h[k] = 1000.0;
+      } else {  // geostrophically balanced height, g h0 = 2.94e4 m2/s2
+        const double t = -cos(lon) * cl * sin(alpha) + sl * cos(alpha);
+        h[k] = 2.94e4 / g - (a * om * u0 + 0.5 * u0 * u0) * t * t / g;
+      }
+    } else if (which == LMW_SETUP_WILLIAMSON5) {
+      // test case 2's zonal flow (u0 = 20 m/s, h0 = 5960 m) over a 2000 m cone of radius pi/9 at (270 E, 30 N); the
+      // fluid depth is the balanced free surface minus the mountain
+      const double u0 = 20.0, R = pi / 9, lonc = 1.5 * pi, latc = pi / 6;
+      const double rr = sqrt(fmin(R * R, (lon - lonc) * (lon - lonc) + (lat - latc) * (lat - latc)));
+      const double mount = 2000.0 * (1 - rr / R);
+      hs[(static_cast<size_t>(b0) + bl) * plane + row + k] = mount;
+      u[k] = u0 * cl;
+      v[k] = 0.0;
+      h[k] = 5960.0 - (a * om * u0 + 0.5 * u0 * u0) * sl * sl / g - mount;
+    } else {  // test case 6: Rossby-Haurwitz wave of wavenumber R = 4, w = K = 7.848e-6 1/s, h0 = 8000 m
+      const double w = 7.848e-6, Kc = 7.848e-6, R = 4;
+      const double c2 = cl * cl, cR1 = pow(cl, R - 1), cR = pow(cl, R), c2R = pow(cl, 2 * R);
+      u[k] = a * w * cl + a * Kc * cR1 * (R * sl * sl - c2) * cos(R * lon);
+      v[k] = -a * Kc * R * cR1 * sl * sin(R * lon);
+      const double A = w / 2 * (2 * om + w) * c2 + Kc * Kc / 4 * c2R * ((R + 1) * c2 + (2 * R * R - R - 2) - 2 * R * R / c2);
+      const double B = 2 * (om + w) * Kc / ((R + 1) * (R + 2)) * cR * ((R * R + 2 * R + 2) - (R + 1) * (R + 1) * c2);
+      const double C = Kc * Kc / 4 * c2R * ((R + 1) * c2 - (R + 2));
+      h[k] = 8000.0 + a * a / g * (A + B * cos(R * lon) + C * cos(2 * R * lon));
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------------------- the step kernels
 
 // Inverse Legendre transform of (zeta, delta, phi) to the Fourier coefficients of U, V, zeta, phi.  MB members per
@@ -1011,10 +1065,68 @@ void SphereCore::set_state_grid(int b, const double *h, const double *u, const d
   in3.upload(h, plane, 0, stream);
   in3.upload(u, plane, plane, stream);
   in3.upload(v, plane, 2 * plane, stream);
-  launch_prep(PREP_STATE, b, 1);
-  launch_anal(ANAL_STATE, 0, 0.0, b, 1, nullptr);
-  t[b] = 0;
-  mark_reference(b);
+  state_from_in3(b, 1);
+}
+
+// in3 large enough for the input grids of nb members
+void SphereCore::ensure_in3(int nb) {
+  const size_t need = static_cast<size_t>(nb) * 3 * J * K;
+  if (in3.n < need) {
+    sync();  // in3 may still be read by queued work
+    in3.alloc(need);
+  }
+}
+
+// The grids h, u, v of members b0 .. b0 + nb - 1 in in3 ([local member][3][J][K]) become their spectral state:
+// set_state_grid (sphere.cpp:217-234) for nb members in one launch of each kernel.
+void SphereCore::state_from_in3(int b0, int nb) {
+  launch_prep(PREP_STATE, b0, nb);
+  launch_anal(ANAL_STATE, 0, 0.0, b0, nb, nullptr);
+  for (int b = b0; b < b0 + nb; b++) t[b] = 0;
+  mark_reference_range(b0, nb);
+}
+
+// set_state_grid of members b0 .. b0 + nb - 1 from host grids [nb][J][K], kSetupChunk members per launch
+void SphereCore::set_state_grids(int b0, int nb, const double *h, const double *u, const double *v) {
+  const size_t plane = static_cast<size_t>(J) * K, pb = plane * sizeof(double);
+  for (int c0 = 0; c0 < nb; c0 += kSetupChunk) {
+    const int n = std::min(kSetupChunk, nb - c0);
+    ensure_in3(n);
+    const double *src[3] = {h, u, v};
+    for (int f = 0; f < 3; f++)
+      LMW_HIP(hipMemcpy2DAsync(in3.p + f * plane, 3 * pb, src[f] + static_cast<size_t>(c0) * plane, pb, pb, n, hipMemcpyHostToDevice, stream));
+    state_from_in3(b0 + c0, n);
+    sync();  // the next chunk reuses in3
+  }
+}
+
+// The analytic set-ups of Williamson et al. (1992) (cpp/main.cpp:174-291) for members b0 .. b0 + nb - 1, evaluated
+// and analysed on the device.  Test case 1 also sets the (single-member) tracer and the advect-only mode.
+void SphereCore::setup(int b0, int nb, int which, double alpha) {
+  if (which != LMW_SETUP_WILLIAMSON1 && which != LMW_SETUP_WILLIAMSON2 && which != LMW_SETUP_WILLIAMSON5 && which != LMW_SETUP_WILLIAMSON6)
+    invalid("unknown set-up: LMW_SETUP_WILLIAMSON1, 2, 5 or 6");
+  if (!std::isfinite(alpha)) invalid("angle must be a finite number");
+  if (which == LMW_SETUP_WILLIAMSON1 && B != 1) invalid("a tracer needs a single-member sphere");
+  const bool tilted = which == LMW_SETUP_WILLIAMSON1 || which == LMW_SETUP_WILLIAMSON2;
+  for (int c0 = 0; c0 < nb; c0 += kSetupChunk) {
+    const int n = std::min(kSetupChunk, nb - c0);
+    for (int b = b0 + c0; b < b0 + c0 + n; b++) {
+      if (which == LMW_SETUP_WILLIAMSON5) has_orography[b] = 1;  // before the flags update of set_axis_tilt
+      set_axis_tilt(b, tilted ? alpha : 0.0);
+    }
+    ensure_in3(n);
+    hipLaunchKernelGGL(k_setup, dim3(J, n), dim3(256), 0, stream, geo(), which, alpha, mu.p, in3.p, hs.p, b0 + c0);
+    LMW_LAUNCH_CHECK();
+    state_from_in3(b0 + c0, n);
+    sync();
+  }
+  if (which == LMW_SETUP_WILLIAMSON1) {
+    hipLaunchKernelGGL(k_setup, dim3(J, 1), dim3(256), 0, stream, geo(), SETUP_TRACER, alpha, mu.p, in3.p, hs.p, 0);
+    LMW_LAUNCH_CHECK();
+    tracer_from_in3();
+    set_advect_only(true);
+    sync();
+  }
 }
 
 void SphereCore::set_spectral_packed(int b, int field, const double2 *packed) {
@@ -1042,12 +1154,15 @@ void SphereCore::get_spectral_packed(int b, int field, double2 *packed) {
   sync();
 }
 
-void SphereCore::mark_reference(int b) {
-  const size_t plane = static_cast<size_t>(J) * K;
-  launch_synth(x0.p, b, 1);
-  launch_to_grid(GRID_RAW, grid.p, b, 1);  // rows of member b land at grid[b][0..3]
-  LMW_HIP(hipMemcpyAsync(ref_phi.p + b * plane, grid.p + (static_cast<size_t>(b) * 4 + 3) * plane, plane * sizeof(double),
-                         hipMemcpyDeviceToDevice, stream));
+void SphereCore::mark_reference(int b) { mark_reference_range(b, 1); }
+
+// mark_reference (sphere.cpp:236-239) of members b0 .. b0 + nb - 1: the phi grids become the reference of error_l2
+void SphereCore::mark_reference_range(int b0, int nb) {
+  const size_t plane = static_cast<size_t>(J) * K, pb = plane * sizeof(double);
+  launch_synth(x0.p, b0, nb);
+  launch_to_grid(GRID_RAW, grid.p, b0, nb);  // rows of member b land at grid[b][0..3]
+  LMW_HIP(hipMemcpy2DAsync(ref_phi.p + static_cast<size_t>(b0) * plane, pb, grid.p + (static_cast<size_t>(b0) * 4 + 3) * plane, 4 * pb, pb, nb,
+                           hipMemcpyDeviceToDevice, stream));
   if (has_tracer) tracer_grid_device(*this, ref_q.p);  // sphere.cpp:238
 }
 
@@ -1138,9 +1253,14 @@ void SphereCore::set_tracer_spectral(const double2 *packed) {
 
 void SphereCore::set_tracer_grid(const double *q) {
   if (B != 1) invalid("a tracer needs a single-member sphere");
-  tracer_buffers(*this);
   const size_t plane = static_cast<size_t>(J) * K;
   in3.upload(q, plane, 0, stream);
+  tracer_from_in3();
+}
+
+// the tracer grid in plane 0 of in3 becomes the tracer's spectral state and its reference (sphere.cpp:241-249)
+void SphereCore::tracer_from_in3() {
+  tracer_buffers(*this);
   launch_prep(PREP_SCALAR, 0, 1);
   launch_anal(ANAL_RAW, 0, 0.0, 0, 1, spec_io.p);
   LMW_HIP(hipMemcpyAsync(q0.p, spec_io.p + 3 * static_cast<size_t>(S), S * sizeof(double2), hipMemcpyDeviceToDevice, stream));

@@ -147,6 +147,11 @@ class Sphere:
     def tracer_at(self, lat: float, lon: float) -> float:
         return self._point(L.POINT_TRACER, lat, lon)
 
+    def setup(self, which: int, alpha: float = 0.0) -> None:
+        """A Williamson et al. (1992) test case (L.SETUP_WILLIAMSON1/2/5/6: the williamson1(alpha), williamson2(alpha),
+        williamson5(), williamson6() operators of main.cpp:174-291), evaluated and analysed on the device."""
+        L.call("lmw_sphere_setup", self._h, int(which), float(alpha))
+
     def add_mode(self, n: int, m: int, amplitude: float, mean_depth: float) -> None:
         L.call("lmw_sphere_add_mode", self._h, n, m, amplitude, mean_depth)
 
@@ -348,6 +353,16 @@ class Ensemble:
     def set_state_grid(self, member, h, u, v) -> None:
         h, u, v = (L.as_f64(x).ravel() for x in (h, u, v))
         L.call("lmw_ensemble_set_state_grid", self._h, member, L.dptr(h), L.dptr(u), L.dptr(v), h.size)
+
+    def set_state_grids(self, member0: int, count: int, h, u, v) -> None:
+        """set_state_grid of members member0 .. member0 + count - 1 (one truncation) from (count, nlat, nlon) arrays,
+        in batched launches."""
+        h, u, v = (L.as_f64(x).ravel() for x in (h, u, v))
+        L.call("lmw_ensemble_set_state_grids", self._h, int(member0), int(count), L.dptr(h), L.dptr(u), L.dptr(v), h.size)
+
+    def setup(self, member0: int, count: int, which: int, alpha: float = 0.0) -> None:
+        """Sphere.setup for members member0 .. member0 + count - 1 (test case 1 needs a tracer: single spheres only)."""
+        L.call("lmw_ensemble_setup", self._h, int(member0), int(count), int(which), float(alpha))
 
     def set_orography_grid(self, member, hs) -> None:
         hs = L.as_f64(hs).ravel()
