@@ -182,7 +182,7 @@ struct MemberView {
     }
     if (field == LMW_GRID_OROGRAPHY) {  // the stored hs grid (sphere.hpp:50)
       const size_t plane = static_cast<size_t>(c->J) * c->K;
-      c->hs.download(out, plane, static_cast<size_t>(b) * plane, c->stream);
+      c->hs.download(out, plane, static_cast<size_t>(c->hs_slot[b]) * plane, c->stream);
       c->sync();
       return;
     }

@@ -63,6 +63,13 @@ struct SphereCore {
   std::vector<double> t;                      // model time
   std::vector<double> tilt;                   // axis tilt
   std::vector<char> has_orography;
+  // orography planes: member b reads plane hs_slot[b] of hs.  Members given bit-identical orography share one
+  // plane (set_orography compares contents; test case 5's cone is one plane per group), so a batch over one terrain
+  // reads it once.  hs_refs counts the members of each plane; hs_kind: 0 unknown, 1 set from the host (hs_key its
+  // hash), 2 test case 5's cone.
+  std::vector<int> hs_slot, hs_refs;
+  std::vector<char> hs_kind;
+  std::vector<uint64_t> hs_key;
   // device
   DevArray<int> moff, poff, kpe, kpo;
   DevArray<double> mu, w;
@@ -82,7 +89,7 @@ struct SphereCore {
   DevArray<double> in3;                        // [b][3][J][K] input grids for the analysis path
   DevArray<double2> spec_io;                   // [b][4][S] raw transform outputs / inputs
   DevArray<double> red;                        // [b][16] reductions
-  DevArray<int> mflags;                        // [b] bit 0 untilted axis, bit 1 no orography
+  DevArray<int> mflags;                        // [b] bit 0 untilted axis, bit 1 no orography, bits 2.. hs plane
   DevArray<float2> Fs32, Gb32;                 // FP32 storage of Fs and Gb for the mixed-precision step
   // passive tracer q (single-member spheres only): state, RK4 stage state and accumulator [S], the spectral input
   // of its transforms [3][S], the delta grid scratch [4][J][K] and the reference tracer grid [J][K]
@@ -108,6 +115,8 @@ struct SphereCore {
   void set_axis_tilt(int b, double alpha);
   void update_flags(int b);
   void set_orography(int b, const double *hs_host);
+  void point_orography(int b, int plane);
+  int own_orography(int b);
   void set_state_grid(int b, const double *h, const double *u, const double *v);
   void set_state_grids(int b0, int nb, const double *h, const double *u, const double *v);
   void setup(int b0, int nb, int which, double alpha);

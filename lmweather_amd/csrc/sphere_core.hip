@@ -165,10 +165,11 @@ __global__ void k_coriolis(Geo G, const double *mu, double *f, double alpha) {
 
 // Analytic initial states of Williamson et al. (1992, J. Comput. Phys. 102:211-224) at the Gauss grid: the set-up
 // operators of cpp/main.cpp:174-291.  in3[local member][h, u, v][j][k] for the members of blockIdx.y; test case 5 also
-// writes its conical mountain into hs[member], and SETUP_TRACER writes test case 1's cosine bell into plane 0 of in3.
+// writes its conical mountain into the orography plane hs_plane (when given), and SETUP_TRACER writes test case 1's
+// cosine bell into plane 0 of in3.
 // lat = asin(mu_j), lon = 2 pi k / K (sphere.cpp:120-121).
 __global__ void k_setup(Geo G, int which, double alpha, const double *__restrict__ mu, double *__restrict__ in3,
-                        double *__restrict__ hs, int b0) {
+                        double *__restrict__ hs_plane) {
   const int j = blockIdx.x, bl = blockIdx.y, K = G.K;
   const size_t plane = static_cast<size_t>(G.J) * K, row = static_cast<size_t>(j) * K;
   const double pi = 3.14159265358979323846, a = G.a, om = G.omega, g = G.g;
@@ -200,7 +201,7 @@ __global__ void k_setup(Geo G, int which, double alpha, const double *__restrict
       const double u0 = 20.0, R = pi / 9, lonc = 1.5 * pi, latc = pi / 6;
       const double rr = sqrt(fmin(R * R, (lon - lonc) * (lon - lonc) + (lat - latc) * (lat - latc)));
       const double mount = 2000.0 * (1 - rr / R);
-      hs[(static_cast<size_t>(b0) + bl) * plane + row + k] = mount;
+      if (hs_plane && bl == 0) hs_plane[row + k] = mount;  // one plane, shared by the members of the launch
       u[k] = u0 * cl;
       v[k] = 0.0;
       h[k] = 5960.0 - (a * om * u0 + 0.5 * u0 * u0) * sl * sl / g - mount;
@@ -289,7 +290,8 @@ __global__ __launch_bounds__(64 * MB) void k_synth(Geo G, const double *__restri
 // Grid-point work of one latitude pair: inverse FFTs, nonlinear products, forward FFTs, analysis operands.
 // Rows in LDS: inverse U, V, zeta, phi and forward A, B, U phi, V phi, phi + g hs + E, each at (north, south).
 // Storage TS of the Fourier coefficients and analysis operands: double2 (FP64 path) or float2 (mixed precision).
-// mflags[b]: bit 0 = rotation axis untilted (f = 2 Omega mu computed, not read), bit 1 = no orography (hs = 0).
+// mflags[b]: bit 0 = rotation axis untilted (f = 2 Omega mu computed, not read), bit 1 = no orography (hs = 0),
+// bits 2.. = the member's plane of hs (members with the same orography share one).
 // Grid-point part of one RK4 stage for one latitude pair and member: inverse FFTs of U, V, zeta, phi (north and
 // south rows), the nonlinear products (sphere.cpp:348-358), forward FFTs of the five products and the analysis
 // operands.  All global loads a thread needs (coefficients, twiddles, orography, f) are issued before the first
@@ -314,15 +316,17 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   const int k = tid;  // one pair of longitudes (2k, 2k+1) per thread in the grid products: L <= 256
   const int flags = mflags[b];
   const bool untilted = flags & 1, flat = flags & 2;
+  const size_t hplane = static_cast<size_t>(flags >> 2);  // this member's orography plane
   // orography and f at this thread's longitudes of both rows
   T hsv[2][2], fv[2][2];
 #pragma unroll
   for (int s = 0; s < 2; s++) {
     const size_t gi = (static_cast<size_t>(b) * G.J + (s ? js : jn)) * K + 2 * k;
+    const size_t gh = (hplane * G.J + (s ? js : jn)) * K + 2 * k;
 #pragma unroll
     for (int e = 0; e < 2; e++) {
       hsv[s][e] = fv[s][e] = T(0);
-      if (k < L && !flat) hsv[s][e] = static_cast<T>(hs[gi + e]);
+      if (k < L && !flat) hsv[s][e] = static_cast<T>(hs[gh + e]);
       if (k < L && !untilted) fv[s][e] = static_cast<T>(fgrid[gi + e]);
     }
   }
@@ -620,7 +624,7 @@ struct Kahan {
 // red[b][0..8] = kinetic, potential, potential enstrophy, max wind, max h, err num, err den, max |h - ref|, max |ref|.
 __global__ __launch_bounds__(256) void k_reduce(Geo G, const double *__restrict__ grid, const double *__restrict__ fgrid,
                                                 const double *__restrict__ hs, const double *__restrict__ ref_phi,
-                                                const double *__restrict__ w, double *__restrict__ red, int b0) {
+                                                const double *__restrict__ w, double *__restrict__ red, int b0, const int *__restrict__ mflags) {
   const int b = b0 + blockIdx.x, j = threadIdx.x, K = G.K;
   __shared__ double rows[9][256];
   const size_t plane = static_cast<size_t>(G.J) * K;
@@ -628,7 +632,7 @@ __global__ __launch_bounds__(256) void k_reduce(Geo G, const double *__restrict_
     const double *hh = grid + static_cast<size_t>(b) * 4 * plane + static_cast<size_t>(j) * K;
     const double *uu = hh + plane, *vv = hh + 2 * plane, *zz = hh + 3 * plane;
     const double *ff = fgrid + static_cast<size_t>(b) * plane + static_cast<size_t>(j) * K;
-    const double *ss = hs + static_cast<size_t>(b) * plane + static_cast<size_t>(j) * K;
+    const double *ss = hs + static_cast<size_t>(mflags[b] >> 2) * plane + static_cast<size_t>(j) * K;
     const double *rr = ref_phi + static_cast<size_t>(b) * plane + static_cast<size_t>(j) * K;
     Kahan ke, pe, en, num, den;
     double wmax = 0, hmax = 0, mn = 0, mr = 0;
@@ -868,6 +872,11 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
   t.assign(B, 0.0);
   tilt.assign(B, 0.0);
   has_orography.assign(B, 0);
+  hs_slot.resize(B);
+  for (int b = 0; b < B; b++) hs_slot[b] = b;
+  hs_refs.assign(B, 1);
+  hs_kind.assign(B, 0);
+  hs_key.assign(B, 0);
   mflags.alloc(B);
   Geo G = geo();
   hipLaunchKernelGGL(k_gauss, dim3(1), dim3(256), 0, stream, J, mu.p, w.p);
@@ -918,7 +927,7 @@ Geo SphereCore::geo() const {
 void SphereCore::sync() { LMW_HIP(hipStreamSynchronize(stream)); }
 
 void SphereCore::update_flags(int b) {
-  const int fl = (tilt[b] == 0.0 ? 1 : 0) | (has_orography[b] ? 0 : 2);
+  const int fl = (tilt[b] == 0.0 ? 1 : 0) | (has_orography[b] ? 0 : 2) | (hs_slot[b] << 2);
   mflags.upload(&fl, 1, b, stream);
   sync();  // the source is a stack variable
 }
@@ -931,9 +940,52 @@ void SphereCore::set_axis_tilt(int b, double alpha) {
   LMW_LAUNCH_CHECK();
 }
 
+void SphereCore::point_orography(int b, int p) {
+  hs_refs[hs_slot[b]]--;
+  hs_slot[b] = p;
+  hs_refs[p]++;
+}
+
+// A plane that member b alone refers to, its contents unknown.  One exists: there are B planes and the other B - 1
+// members refer to at most B - 1 of them.
+int SphereCore::own_orography(int b) {
+  int p = hs_slot[b];
+  if (hs_refs[p] != 1) {
+    for (p = 0; p < B && hs_refs[p] != 0; p++) {
+    }
+    point_orography(b, p);
+  }
+  hs_kind[p] = 0;
+  return p;
+}
+
+// set_orography_grid (sphere.cpp:105-109): member b shares a plane that already holds the same bits (checked on the
+// device copy), else gets one of its own
 void SphereCore::set_orography(int b, const double *hsh) {
   const size_t plane = static_cast<size_t>(J) * K;
-  hs.upload(hsh, plane, b * plane, stream);
+  uint64_t key = 1469598103934665603ull;  // FNV-1a over the 64-bit words
+  for (size_t i = 0; i < plane; i++) {
+    uint64_t bits;
+    std::memcpy(&bits, hsh + i, sizeof bits);
+    key = (key ^ bits) * 1099511628211ull;
+  }
+  int found = -1;
+  std::vector<double> tmp;
+  for (int p = 0; p < B && found < 0; p++) {
+    if (hs_kind[p] != 1 || hs_key[p] != key) continue;
+    tmp.resize(plane);
+    hs.download(tmp.data(), plane, static_cast<size_t>(p) * plane, stream);
+    sync();
+    if (std::memcmp(tmp.data(), hsh, plane * sizeof(double)) == 0) found = p;
+  }
+  if (found >= 0) {
+    point_orography(b, found);
+  } else {
+    const int p = own_orography(b);
+    hs.upload(hsh, plane, static_cast<size_t>(p) * plane, stream);
+    hs_kind[p] = 1;
+    hs_key[p] = key;
+  }
   has_orography[b] = 1;
   update_flags(b);
 }
@@ -1108,20 +1160,32 @@ void SphereCore::setup(int b0, int nb, int which, double alpha) {
   if (!std::isfinite(alpha)) invalid("angle must be a finite number");
   if (which == LMW_SETUP_WILLIAMSON1 && B != 1) invalid("a tracer needs a single-member sphere");
   const bool tilted = which == LMW_SETUP_WILLIAMSON1 || which == LMW_SETUP_WILLIAMSON2;
+  const size_t plane = static_cast<size_t>(J) * K;
   for (int c0 = 0; c0 < nb; c0 += kSetupChunk) {
     const int n = std::min(kSetupChunk, nb - c0);
-    for (int b = b0 + c0; b < b0 + c0 + n; b++) {
-      if (which == LMW_SETUP_WILLIAMSON5) has_orography[b] = 1;  // before the flags update of set_axis_tilt
-      set_axis_tilt(b, tilted ? alpha : 0.0);
+    double *cone = nullptr;  // the plane this launch writes test case 5's cone into, if no plane holds it yet
+    if (which == LMW_SETUP_WILLIAMSON5) {
+      int p = 0;
+      while (p < B && !(hs_kind[p] == 2 && hs_refs[p] > 0)) p++;
+      if (p == B) {
+        p = own_orography(b0 + c0);
+        hs_kind[p] = 2;
+        cone = hs.p + static_cast<size_t>(p) * plane;
+      }
+      for (int b = b0 + c0; b < b0 + c0 + n; b++) {
+        point_orography(b, p);
+        has_orography[b] = 1;  // before the flags update of set_axis_tilt
+      }
     }
+    for (int b = b0 + c0; b < b0 + c0 + n; b++) set_axis_tilt(b, tilted ? alpha : 0.0);
     ensure_in3(n);
-    hipLaunchKernelGGL(k_setup, dim3(J, n), dim3(256), 0, stream, geo(), which, alpha, mu.p, in3.p, hs.p, b0 + c0);
+    hipLaunchKernelGGL(k_setup, dim3(J, n), dim3(256), 0, stream, geo(), which, alpha, mu.p, in3.p, cone);
     LMW_LAUNCH_CHECK();
     state_from_in3(b0 + c0, n);
     sync();
   }
   if (which == LMW_SETUP_WILLIAMSON1) {
-    hipLaunchKernelGGL(k_setup, dim3(J, 1), dim3(256), 0, stream, geo(), SETUP_TRACER, alpha, mu.p, in3.p, hs.p, 0);
+    hipLaunchKernelGGL(k_setup, dim3(J, 1), dim3(256), 0, stream, geo(), SETUP_TRACER, alpha, mu.p, in3.p, nullptr);
     LMW_LAUNCH_CHECK();
     tracer_from_in3();
     set_advect_only(true);
@@ -1349,7 +1413,7 @@ void SphereCore::compute_grids(int b0, int nb) {
 }
 
 void SphereCore::reduce(int b0, int nb) {
-  hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(256), 0, stream, geo(), grid.p, f.p, hs.p, ref_phi.p, w.p, red.p, b0);
+  hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(256), 0, stream, geo(), grid.p, f.p, hs.p, ref_phi.p, w.p, red.p, b0, mflags.p);
   LMW_LAUNCH_CHECK();
 }
 

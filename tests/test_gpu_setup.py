@@ -130,3 +130,43 @@ def test_set_state_grids_matches_one_member_path(lmw, precision):
         a.set_state_grids(0, 2, h[:1], u[:1], v[:1])
     with pytest.raises(ValueError, match="^member index out of range$"):
         a.set_state_grids(count, 3, h[:3], u[:3], v[:3])
+
+
+def test_orography_planes_shared_and_private(lmw):
+    """Members with bit-identical orography share one device plane; setting a member's orography never changes
+    another member's, whichever plane they shared (each member's orography, energy and tendencies equal those of a
+    one-member sphere given the same state and orography)."""
+    from lmweather_amd.initial import gaussian_orography, random_balanced_state
+    N = 21
+    base = sphere(lmw, N)
+    h, u, v = random_balanced_state(base, 8, 3)
+    ta, tb, tc = (gaussian_orography(base, 4, seed) for seed in (5, 6, 7))
+    e = lmw.Ensemble([N] * 4, lmw.SphereParams(A, OM, G))
+    for i in range(4):
+        e.set_state_grid(i, h, u, v)
+    plan = [ta, ta, ta, None]  # member 3 stays flat
+    for i, t in enumerate(plan):
+        if t is not None:
+            e.set_orography_grid(i, t)
+    e.set_orography_grid(1, tb)  # leaves a shared plane
+    e.set_orography_grid(0, tb)  # joins member 1's plane
+    e.set_orography_grid(2, tc)  # sole user of the first plane: rewritten in place
+    e.set_orography_grid(1, ta)
+    plan = [tb, ta, tc, None]
+    e.setup(3, 1, SU.W5, 0.0)  # the cone: a plane of its own
+    e.set_orography_grid(0, tc)  # shares member 2's plane
+    plan = [tc, ta, tc, "cone"]
+    cone = None
+    for i, t in enumerate(plan):
+        s = sphere(lmw, N)
+        if isinstance(t, str):
+            s.setup(SU.W5, 0.0)
+            cone = s
+        else:
+            s.set_orography_grid(t)
+            s.set_state_grid(h, u, v)
+        assert np.array_equal(e.grid(i, 5), s.grid_orography()), i
+        for field in range(3):
+            assert np.array_equal(e.get_spectral(i, field), s.get_spectral(field)), (i, field)
+        assert e.diag(i, 3) == s.potential_energy(), i  # potential energy reads the orography
+    assert cone is not None
