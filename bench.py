@@ -324,9 +324,22 @@ def setup_bench(lmw, args):
         "host_grid_bytes": int(H.nbytes * 3),
     }
     del ens
+    # the seeded plane perturbation (plane.cpp:34-57) of 64 members of the 1024 x 512 channel, on the device
+    pl = lmw.Plane(lmw.PlaneParams(1024, 512, 1.024e7, 5.12e6, 9.81, 1000.0, 1e-4, 1.6e-11), 64, lmw.FP64)
+    pl.perturb(1.0, 1, 0)
+    t0 = time.perf_counter()
+    for b in range(64):
+        pl.perturb(1.0, 100 + b, b)
+    out["plane_perturb_1024x512_per_s"] = 64 / (time.perf_counter() - t0)
+    del pl
     if not args.no_cpu:
         from oracle import pyapi
         kind = "reference" if pyapi.available("ref") else "port"
+        op = pyapi.OraclePlane(1024, 512, 1.024e7, 5.12e6, 9.81, 1000.0, 1e-4, 1.6e-11, kind="ref" if kind == "reference" else "oracle")
+        t0 = time.perf_counter()
+        for b in range(3):
+            op.perturb(1.0, 100 + b)
+        out["cpu_plane_perturb_1024x512_per_s"] = 3 / (time.perf_counter() - t0)
         s = pyapi.OracleSphere(N, A, OM, G, kind="ref" if kind == "reference" else "oracle")
         s.set_state_grid(h, u, v)
         n = 10

@@ -95,39 +95,7 @@ lmw_status lmw_plane_perturb(lmw_plane *p, int member, double amplitude, uint64_
   return guard([&] {
     lmw_plane &pl = P(p);
     check_member(pl, member);
-    PlaneCoreBase &c = *pl.core;
-    // plane.cpp:34-57: seeded noise, three 1-2-1 passes, zero mean, scaled to the rms -- host set-up
-    const int nx = c.prm.nx, ny = c.prm.ny;
-    const size_t nh = c.nh();
-    uint64_t st = seed * 0x9E3779B97F4A7C15ULL + 777;
-    std::vector<double> n(nh), t(nh), h(nh);
-    for (auto &x : n) {
-      double s = 0;
-      for (int i = 0; i < 12; i++) s += static_cast<double>(splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
-      x = s - 6.0;
-    }
-    for (int pass = 0; pass < 3; pass++) {
-      for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++)
-          t[static_cast<size_t>(j) * nx + i] = 0.25 * n[static_cast<size_t>(j) * nx + (i + nx - 1) % nx] + 0.5 * n[static_cast<size_t>(j) * nx + i] +
-                                               0.25 * n[static_cast<size_t>(j) * nx + (i + 1) % nx];
-      for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++)
-          n[static_cast<size_t>(j) * nx + i] = 0.25 * t[static_cast<size_t>(std::max(j - 1, 0)) * nx + i] + 0.5 * t[static_cast<size_t>(j) * nx + i] +
-                                               0.25 * t[static_cast<size_t>(std::min(j + 1, ny - 1)) * nx + i];
-    }
-    double mean = 0, ms = 0;
-    for (double x : n) mean += x;
-    mean /= n.size();
-    for (double &x : n) {
-      x -= mean;
-      ms += x * x;
-    }
-    const double rms = std::sqrt(ms / n.size());
-    const double scale = rms > 0 ? amplitude / rms : 0;
-    c.get_field(member, 0, h.data());
-    for (size_t i = 0; i < nh; i++) h[i] += n[i] * scale;
-    c.set_field(member, 0, h.data());
+    pl.core->perturb(member, amplitude, seed);
   });
 }
 lmw_status lmw_plane_set_field(lmw_plane *p, int member, int field, const double *g, size_t n) {

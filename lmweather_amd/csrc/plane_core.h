@@ -647,11 +647,46 @@ template <class T> __global__ void k_widen(const T *__restrict__ src, double *__
   if (i < n) dst[i] = static_cast<double>(src[i]);
 }
 
-uint64_t splitmix(uint64_t &x) {
-  uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+// The seeded noise of perturb (plane.cpp:34-57) on the device.  splitmix64 advances its state by a constant, so the
+// k-th draw of the stream is mix(st0 + k gamma) and every cell's twelve draws (12 c + 1 .. 12 c + 12) are computed
+// independently; the twelve terms are summed in draw order, as the sequential stream sums them, so the noise is bit
+// for bit the reference's.  These kernels keep contraction off in both precisions: the perturbation is the same
+// double-precision field before the state's rounding to T.
+__device__ inline uint64_t splitmix_at(uint64_t st0, uint64_t k) {
+  uint64_t z = st0 + k * 0x9E3779B97F4A7C15ULL;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
   return z ^ (z >> 31);
+}
+__global__ void k_plane_noise(uint64_t st0, size_t cells, double *__restrict__ n) {
+  #pragma clang fp contract(off)
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  double s = 0;
+  for (int k = 0; k < 12; k++) s += static_cast<double>(splitmix_at(st0, 12 * c + k + 1) >> 11) * (1.0 / 9007199254740992.0);
+  n[c] = s - 6.0;
+}
+// One 1-2-1 pass along x (periodic; dir 0) or along y (the edge rows repeated; dir 1) of an ny x nx field.
+__global__ void k_plane_smooth(int nx, int ny, int dir, const double *__restrict__ in, double *__restrict__ out) {
+  #pragma clang fp contract(off)
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= static_cast<size_t>(nx) * ny) return;
+  const int j = static_cast<int>(c / nx), i = static_cast<int>(c - static_cast<size_t>(j) * nx);
+  double a, z;
+  if (dir == 0) {
+    a = in[c - i + (i == 0 ? nx - 1 : i - 1)];
+    z = in[c - i + (i == nx - 1 ? 0 : i + 1)];
+  } else {
+    a = in[j == 0 ? c : c - nx];
+    z = in[j == ny - 1 ? c : c + nx];
+  }
+  out[c] = 0.25 * a + 0.5 * in[c] + 0.25 * z;
+}
+// h += scale (n - mean), in double, rounded to T once
+template <class T> __global__ void k_plane_add_noise(T *__restrict__ h, const double *__restrict__ n, double mean, double scale, size_t cells) {
+  #pragma clang fp contract(off)
+  const size_t c = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c < cells) h[c] = static_cast<T>(static_cast<double>(h[c]) + (n[c] - mean) * scale);
 }
 
 }  // namespace
@@ -692,6 +727,8 @@ struct PlaneCoreBase {
   virtual void harmonic(int b, int field, int m, int shift2, int j0, int rows, double *out) = 0;
   virtual void set_tile(int tile) = 0;
   virtual void reset() = 0;
+  // perturb (plane.cpp:34-57) of member b's h
+  virtual void perturb(int b, double amplitude, uint64_t seed) = 0;
   void sync() { LMW_HIP(hipStreamSynchronize(stream)); }
 };
 
@@ -725,6 +762,37 @@ template <class T> struct PlaneCore : PlaneCoreBase {
   }
   size_t field_off(int b, int field) const { return static_cast<size_t>(b) * per() + (field == 0 ? 0 : field == 1 ? nh() : 2 * nh()); }
   size_t field_len(int field) const { return field == 2 ? nv() : nh(); }
+  DevArray<double> noise;  // [2][ny][nx]: the noise field and the smoothing passes' other buffer
+  void perturb(int b, double amplitude, uint64_t seed) override {
+    const size_t cells = nh();
+    if (noise.n < 2 * cells) noise.alloc(2 * cells);
+    const unsigned blocks = static_cast<unsigned>((cells + 255) / 256);
+    double *n = noise.p, *t = noise.p + cells;
+    hipLaunchKernelGGL(k_plane_noise, dim3(blocks), dim3(256), 0, stream, seed * 0x9E3779B97F4A7C15ULL + 777, cells, n);
+    LMW_LAUNCH_CHECK();
+    for (int pass = 0; pass < 3; pass++) {
+      hipLaunchKernelGGL(k_plane_smooth, dim3(blocks), dim3(256), 0, stream, prm.nx, prm.ny, 0, n, t);
+      LMW_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_plane_smooth, dim3(blocks), dim3(256), 0, stream, prm.nx, prm.ny, 1, t, n);
+      LMW_LAUNCH_CHECK();
+    }
+    // the mean and the rms in the reference's sequential order (host), so the scale is its scale bit for bit
+    std::vector<double> hn(cells);
+    noise.download(hn.data(), cells, 0, stream);
+    sync();
+    double mean = 0, ms = 0;
+    for (double x : hn) mean += x;
+    mean /= static_cast<double>(cells);
+    for (double x : hn) {
+      const double d = x - mean;
+      ms += d * d;
+    }
+    const double rms = std::sqrt(ms / static_cast<double>(cells));
+    const double scale = rms > 0 ? amplitude / rms : 0;
+    hipLaunchKernelGGL(k_plane_add_noise<T>, dim3(blocks), dim3(256), 0, stream, x0.p + field_off(b, 0), n, mean, scale, cells);
+    LMW_LAUNCH_CHECK();
+    sync();
+  }
   void reset() override {
     // h = depth, u = v = 0 (plane.cpp:27-32)
     std::vector<double> hh(nh(), prm.depth), zero(nv(), 0.0);

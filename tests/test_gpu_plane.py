@@ -154,3 +154,23 @@ def test_fused_step_tiles_bit_identical(lmw, oracle, precision, tiles, physics):
         for f in range(3):
             ref = q.get_field(f)
             assert np.max(np.abs(outs[0][f] - ref)) <= 1e-12 * np.max(np.abs(ref)), f
+
+
+@pytest.mark.parametrize("precision,nx,ny", [(0, 64, 32), (0, 1024, 512), (1, 200, 48)])
+def test_perturb_bit_identical(lmw, oracle, precision, nx, ny):
+    """perturb (plane.cpp:34-57) generates the seeded noise and smooths it on the device; the result is the
+    reference's bit for bit (FP64), and its rounding to FP32 for an FP32 plane.  Members are independent."""
+    g, o = pair(lmw, oracle, nx=nx, ny=ny, lx=nx * 1e4, ly=ny * 1e4, batch=3, precision=precision)
+    g.perturb(0.75, 12345, member=1)
+    o.perturb(0.75, 12345)
+    want = o.get_field(0)
+    got = g.get_field(0, member=1)
+    if precision == 0:
+        assert np.array_equal(got, want)
+    else:
+        assert np.array_equal(got, want.astype(np.float32).astype(np.float64))
+    assert np.all(g.get_field(0, member=0) == 1000.0) and np.all(g.get_field(0, member=2) == 1000.0)
+    g.perturb(0.75, 12345, member=1)  # a second perturbation adds another field of the same stream
+    o.perturb(0.75, 12345)
+    if precision == 0:
+        assert np.array_equal(g.get_field(0, member=1), o.get_field(0))
