@@ -310,7 +310,11 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   extern __shared__ double2 lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
   const int K = 2 * L, M1 = G.M1, tid = threadIdx.x;
-  const int jh = blockIdx.x, b = b0 + blockIdx.y;
+  // XCD-aware order: the hardware deals consecutive blocks to the 8 XCDs in turn, so block x of a member takes latitude
+  // pair (x mod 8) Jh/8 + x/8 and each XCD works through a contiguous band of latitude pairs.  The 32-byte operand
+  // records of neighbouring pairs share cache lines, which then fill in one XCD's L2 before they are written back.
+  const int jh = (G.Jh & 7) ? static_cast<int>(blockIdx.x) : static_cast<int>((blockIdx.x & 7) * (G.Jh >> 3) + (blockIdx.x >> 3));
+  const int b = b0 + blockIdx.y;
   const int jn = G.Jh + jh, js = G.Jh - 1 - jh;
   C *buf = lds, *tw = lds + 10 * P;
   const int k = tid;  // one pair of longitudes (2k, 2k+1) per thread in the grid products: L <= 256
