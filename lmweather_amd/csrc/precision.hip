@@ -556,11 +556,21 @@ __device__ inline bool xcd_item(int items, int tiles, int &item, int &tile) {
   return item < items;
 }
 
+// The inverse transform's order: chunks of 4 consecutive m on one XCD (chunk c on XCD c mod 8), the member tiles of
+// an m 4 block ids apart within its XCD.  Its output records (32 bytes per (member, latitude, m)) of 4 consecutive m
+// share a cache line, which then fills in one L2.  A 1-D grid of ceil(ceil(M1 / 4) / 8) * 8 * 4 * tiles blocks.
+__device__ inline bool xcd_item4(int items, int tiles, int &item, int &tile) {
+  const int x = blockIdx.x & 7, s = blockIdx.x >> 3, w = s % (4 * tiles);
+  item = ((s / (4 * tiles)) * 8 + x) * 4 + (w & 3);
+  tile = w >> 2;
+  return item < items;
+}
+
 template <int PREC, int RC>
 __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
                                                   int nb, int mixed, int tiles) {
   int m, t;
-  if (!xcd_item(G.M1, tiles, m, t)) return;
+  if (!xcd_item4(G.M1, tiles, m, t)) return;
   synth_mx_block<PREC, RC>(G, T, spec, Fs, b0, nb, mixed, m, t * MEM);
 }
 
@@ -679,7 +689,7 @@ static void synth_mx_launch(SphereCore &c, const double2 *spec, int b0, int nb) 
   SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Sp3.p, c.Sh3.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
   const int waves = (c.Jh + 15) / 16;
   const int tiles = (nb + MEM - 1) / MEM;
-  dim3 grd((c.M1 + 7) / 8 * 8 * tiles), blk(64 * waves);
+  dim3 grd(((c.M1 + 3) / 4 + 7) / 8 * 8 * 4 * tiles), blk(64 * waves);
   const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : nsplit(PREC) * 8 * (RC + 8) * MEM * sizeof(__bf16);
   hipLaunchKernelGGL((k_synth_mx<PREC, RC>), grd, blk, lds, c.stream, c.geo(), T, spec, c.Fs32.p, b0, nb, 1, tiles);
   LMW_LAUNCH_CHECK();
