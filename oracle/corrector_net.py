@@ -2,7 +2,7 @@
 pass and Adam.  TEST INFRASTRUCTURE ONLY: the tests import it to check the GPU kernels' outputs and gradients; the
 product path never imports it.
 
-The network has no counterpart in the reference (it is one of the three additions of this build).  Its
+The network has no counterpart in the reference: it is an addition beside the reference's model.  Its
 definition, as implemented by the GPU kernels:
 
 * rows are (sample, packed coefficient (m, n)), C = (N+1)(N+2)/2 per sample, packed index
