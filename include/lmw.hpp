@@ -271,6 +271,12 @@ class Plane {
   double h_at(double x, double y) const { return interp(x, y, 0); }
   double u_at(double x, double y) const { return interp(x, y, 1); }
   double v_at(double x, double y) const { return interp(x, y, 2); }
+  // the Plane.wave_speed and Plane.rossby_radius operators (cpp/main.cpp:445-449), from the parameters
+  double wave_speed() const { return std::sqrt(p_.gravity * p_.depth); }
+  double rossby_radius() const {
+    if (p_.f0 == 0) throw std::invalid_argument("Plane.rossby_radius: f0 is zero");
+    return std::sqrt(p_.gravity * p_.depth) / std::fabs(p_.f0);
+  }
   // zonal harmonics of the device state (cpp/plane.cpp:264-310)
   double zonal_amplitude(int m, int j) const { return harmonic(lmw_plane_zonal_amplitude, m, j); }
   double zonal_crest(int m, int j) const { return harmonic(lmw_plane_zonal_crest, m, j); }

@@ -134,6 +134,14 @@ class Plane:
     def potential_vorticity_anomaly_rms(self, member: int = 0) -> float:
         return self._diag(L.PDIAG_PV_ANOMALY_RMS, member)
 
+    def wave_speed(self) -> float:  # main.cpp:445
+        return math.sqrt(self.params.gravity * self.params.depth)
+
+    def rossby_radius(self) -> float:  # main.cpp:446-449
+        if self.params.f0 == 0:
+            raise ValueError("Plane.rossby_radius: f0 is zero")
+        return math.sqrt(self.params.gravity * self.params.depth) / abs(self.params.f0)
+
     # zonal harmonics (plane.cpp:264-310), on the device
     def _harm(self, name, *args) -> float:
         out = C.c_double()

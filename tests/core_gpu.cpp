@@ -118,6 +118,12 @@ int main() {
     q.linear = true;
     lmw::Plane pl(q);
     const double k = 2 * pi / q.lx, c = std::sqrt(q.gravity * q.depth);
+    CLOSE(pl.wave_speed(), c, 0.0);
+    bool no_f0 = false;
+    try {
+      (void)pl.rossby_radius();
+    } catch (const std::invalid_argument &) { no_f0 = true; }
+    EXPECT(no_f0);
     for (int j = 0; j < q.ny; j++)
       for (int i = 0; i < q.nx; i++) pl.h(i, j) = q.depth + std::cos(k * pl.x_center(i));
     pl.push();
