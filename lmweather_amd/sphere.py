@@ -147,6 +147,14 @@ class Sphere:
     def tracer_at(self, lat: float, lon: float) -> float:
         return self._point(L.POINT_TRACER, lat, lon)
 
+    def autotune(self, force: bool = False):
+        """Choose the Legendre tile shapes for this sphere (timed on its own state, which is restored, or taken from
+        the tile cache); returns (tiles[4], best_ms[2])."""
+        tiles = np.zeros(4, dtype=np.int32)
+        ms = np.zeros(2)
+        L.call("lmw_sphere_autotune", self._h, int(bool(force)), tiles.ctypes.data_as(L._ip), L.dptr(ms))
+        return tiles, ms
+
     def setup(self, which: int, alpha: float = 0.0) -> None:
         """A Williamson et al. (1992) test case (L.SETUP_WILLIAMSON1/2/5/6: the williamson1(alpha), williamson2(alpha),
         williamson5(), williamson6() operators of main.cpp:174-291), evaluated and analysed on the device."""
