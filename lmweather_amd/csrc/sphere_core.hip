@@ -957,6 +957,7 @@ int SphereCore::own_orography(int b) {
   if (hs_refs[p] != 1) {
     for (p = 0; p < B && hs_refs[p] != 0; p++) {
     }
+    if (p == B) runtime("no free orography plane (the plane reference counts are inconsistent)");
     point_orography(b, p);
   }
   hs_kind[p] = 0;
