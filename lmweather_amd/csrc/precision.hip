@@ -62,14 +62,28 @@ __device__ inline v4f mma6(v8bf a1, v8bf a2, v8bf a3, v8bf b1, v8bf b2, v8bf b3,
   return c;
 }
 __device__ inline v8bf load8(const __bf16 *p) { return *reinterpret_cast<const v8bf *>(p); }
+// eight FP32 table entries (32-byte aligned) split into their three BF16 parts, as the tables are split for the
+// operands: 4 bytes per entry from memory instead of the 6 of three stored parts
+__device__ inline void split8(const float *p, v8bf &a1, v8bf &a2, v8bf &a3) {
+  const v4f lo = *reinterpret_cast<const v4f *>(p), hi = *reinterpret_cast<const v4f *>(p + 4);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    __bf16 x1, x2, x3;
+    split_bf16x3(i < 4 ? lo[i] : hi[i - 4], x1, x2, x3);
+    a1[i] = x1;
+    a2[i] = x2;
+    a3[i] = x3;
+  }
+}
 }  // namespace
 
 // ------------------------------------------------------------------------------------------- table builders
 
-// Forward-transform A operand: FP32 [m][k = jh (P) | Jhp + jh (H)][r] and split BF16 [m][r][k], from the FP64
-// analysis tables Pa/Ha [m][jh][r].
+// Forward-transform A operand: FP32 [m][k = jh (P) | Jhp + jh (H)][r], split BF16 parts 1 and 2 [m][r][k] (BF16X3),
+// and FP32 [m][r][k] in that order (Abf: BF16X6 splits it into three parts as it loads), from the FP64 analysis
+// tables Pa/Ha [m][jh][r].
 __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa, const double *__restrict__ Ha, float *__restrict__ A32,
-                                    __bf16 *__restrict__ Abh, __bf16 *__restrict__ Abl, __bf16 *__restrict__ Ab3, const int *__restrict__ mrow) {
+                                    __bf16 *__restrict__ Abh, __bf16 *__restrict__ Abl, float *__restrict__ Abf, const int *__restrict__ mrow) {
   const size_t total = static_cast<size_t>(2 * G.Jhp) * R;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += static_cast<size_t>(gridDim.x) * blockDim.x) {
     // e enumerates (global row gr, k) with k fastest: gr = poff[m] + r
@@ -83,7 +97,7 @@ __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa,
     split_bf16x3(v, x1, x2, x3);  // x1, x2 are also the two-part split
     Abh[e] = x1;
     Abl[e] = x2;
-    Ab3[e] = x3;
+    Abf[e] = v;
   }
 }
 
@@ -97,12 +111,13 @@ __global__ void k_build_anal64(Geo G, const double *__restrict__ Pa, const doubl
   }
 }
 
-// Inverse-transform A operands: FP32 Ps32/Hs32 in the layout of Ps/Hs ([row][jh]), and split BF16 [m][p][jh][k']
-// with each parity block padded to 32 rows (soff[m] is the offset of m, ke32/ko32 the padded row counts).
+// Inverse-transform A operands: FP32 Ps32/Hs32 in the layout of Ps/Hs ([row][jh]), split BF16 parts 1 and 2
+// [m][p][jh][k'] (BF16X3) and FP32 Spf/Shf in that order (BF16X6 splits them as it loads), each parity block
+// padded to 32 rows (soff[m] is the offset of m, ke32/ko32 the padded row counts).
 __global__ void k_build_synth_tables(Geo G, int R, const double *__restrict__ Ps, const double *__restrict__ Hs, float *__restrict__ Ps32,
                                      float *__restrict__ Hs32, const int *__restrict__ soff, const int *__restrict__ ke32,
                                      const int *__restrict__ ko32, __bf16 *__restrict__ Sph, __bf16 *__restrict__ Spl, __bf16 *__restrict__ Shh,
-                                     __bf16 *__restrict__ Shl, __bf16 *__restrict__ Sp3, __bf16 *__restrict__ Sh3) {
+                                     __bf16 *__restrict__ Shl, float *__restrict__ Spf, float *__restrict__ Shf) {
   const size_t total = static_cast<size_t>(R) * G.Jhp;
   for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += static_cast<size_t>(gridDim.x) * blockDim.x) {
     Ps32[e] = static_cast<float>(Ps[e]);
@@ -125,11 +140,11 @@ __global__ void k_build_synth_tables(Geo G, int R, const double *__restrict__ Ps
         split_bf16x3(vp, x1, x2, x3);
         Sph[base + t] = x1;
         Spl[base + t] = x2;
-        Sp3[base + t] = x3;
+        Spf[base + t] = vp;
         split_bf16x3(vh, x1, x2, x3);
         Shh[base + t] = x1;
         Shl[base + t] = x2;
-        Sh3[base + t] = x3;
+        Shf[base + t] = vh;
       }
     }
   }
@@ -139,7 +154,8 @@ __global__ void k_build_synth_tables(Geo G, int R, const double *__restrict__ Ps
 
 struct SynthTabs {
   const float *Ps32, *Hs32;
-  const __bf16 *Sph, *Spl, *Shh, *Shl, *Sp3, *Sh3;
+  const __bf16 *Sph, *Spl, *Shh, *Shl;
+  const float *Spf, *Shf;  // FP32 in the BF16 layout, split into three parts on load (BF16X6)
   const int *soff, *ke32, *ko32;
 };
 
@@ -236,9 +252,10 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
         for (int s = 0; s < nk; s++) {
           const size_t base = static_cast<size_t>(T.soff[m]) + (p ? static_cast<size_t>(G.Jhp) * T.ke32[m] : 0) +
                               static_cast<size_t>(jh) * kp32 + c0 + 32 * s + 8 * (lane >> 4);
-          const v8bf aPh = load8(T.Sph + base), aPl = load8(T.Spl + base), aHh = load8(T.Shh + base), aHl = load8(T.Shl + base);
           if constexpr (PREC == LMW_BF16X6) {
-            const v8bf aP3 = load8(T.Sp3 + base), aH3 = load8(T.Sh3 + base);
+            v8bf aPh, aPl, aP3, aHh, aHl, aH3;
+            split8(T.Spf + base, aPh, aPl, aP3);
+            split8(T.Shf + base, aHh, aHl, aH3);
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
@@ -247,6 +264,7 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
               if (q >= 4) accH[p][q - 4] = mma6(aHh, aHl, aH3, bh, bl, b3, accH[p][q - 4]);
             }
           } else {
+            const v8bf aPh = load8(T.Sph + base), aPl = load8(T.Spl + base), aHh = load8(T.Shh + base), aHl = load8(T.Shl + base);
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
@@ -291,7 +309,8 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
 
 struct AnalTabs {
   const float *A32;
-  const __bf16 *Abh, *Abl, *Ab3;
+  const __bf16 *Abh, *Abl;
+  const float *Abf;  // FP32 in the BF16 layout, split into three parts on load (BF16X6)
   const double *A64;  // FP64 copy of A32 (the FP64 MFMA variant)
 };
 // Operand storage of the forward transform: FP64 operands for the FP64 MFMA variant, FP32 for the mixed modes
@@ -398,15 +417,16 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       } else {
         for (int s = 0; s < kc / 32; s++) {
           const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + r) + c0 + 32 * s + 8 * (lane >> 4);
-          const v8bf ah = load8(T.Abh + ai), al = load8(T.Abl + ai);
           if constexpr (PREC == LMW_BF16X6) {
-            const v8bf a3 = load8(T.Ab3 + ai);
+            v8bf ah, al, a3;
+            split8(T.Abf + ai, ah, al, a3);
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
               accv[ti][q] = mma6(ah, al, a3, load8(bh + bo), load8(blo + bo), load8(b3p + bo), accv[ti][q]);
             }
           } else {
+            const v8bf ah = load8(T.Abh + ai), al = load8(T.Abl + ai);
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
@@ -667,13 +687,13 @@ void SphereCore::build_mixed_tables() {
   Spl.alloc(tot);
   Shh.alloc(tot);
   Shl.alloc(tot);
-  Sp3.alloc(tot);
-  Sh3.alloc(tot);
-  Ab3.alloc(na);
-  hipLaunchKernelGGL(k_build_anal_tables, dim3(1024), dim3(256), 0, stream, G, R, Pa.p, Ha.p, A32.p, Abh.p, Abl.p, Ab3.p, d_mrow.p);
+  Spf.alloc(tot);
+  Shf.alloc(tot);
+  Abf.alloc(na);
+  hipLaunchKernelGGL(k_build_anal_tables, dim3(1024), dim3(256), 0, stream, G, R, Pa.p, Ha.p, A32.p, Abh.p, Abl.p, Abf.p, d_mrow.p);
   LMW_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_build_synth_tables, dim3(M1), dim3(256), 0, stream, G, R, Ps.p, Hs.p, Ps32.p, Hs32.p, d_soff.p, d_ke32.p, d_ko32.p,
-                     Sph.p, Spl.p, Shh.p, Shl.p, Sp3.p, Sh3.p);
+                     Sph.p, Spl.p, Shh.p, Shl.p, Spf.p, Shf.p);
   LMW_LAUNCH_CHECK();
   sync();
   mixed_ready = true;
@@ -686,7 +706,7 @@ const int kAnalNW[kAnalTiles] = {4, 4, 4, 8, 8, 8};
 
 template <int PREC, int RC>
 static void synth_mx_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
-  SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Sp3.p, c.Sh3.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
+  SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Spf.p, c.Shf.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
   const int waves = (c.Jh + 15) / 16;
   const int tiles = (nb + MEM - 1) / MEM;
   dim3 grd(((c.M1 + 3) / 4 + 7) / 8 * 8 * 4 * tiles), blk(64 * waves);
@@ -711,7 +731,7 @@ void SphereCore::launch_synth_mx(const double2 *spec, int b0, int nb) {
 
 template <int PREC, int KC, int NW>
 static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
-  AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p, c.A64.p};
+  AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p, c.Abf.p, c.A64.p};
   const int tiles = (nb + MEM - 1) / MEM;
   dim3 grd((2 * c.M1 + 7) / 8 * 8 * tiles), blk(64 * NW);
   const size_t lds = PREC == LMW_FP64   ? 8 * KC * MEM * sizeof(double)
@@ -824,7 +844,7 @@ static void synth_multi(const std::vector<SphereCore *> &gs, const MultiWork &w,
   int waves = 1;
   for (size_t g = 0; g < gs.size(); g++) {
     SphereCore &c = *gs[g];
-    M.g[g] = SynthGroup{c.geo(), SynthTabs{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Sp3.p, c.Sh3.p, c.d_soff.p, c.d_ke32.p,
+    M.g[g] = SynthGroup{c.geo(), SynthTabs{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Spf.p, c.Shf.p, c.d_soff.p, c.d_ke32.p,
                                              c.d_ko32.p},
                         c.xs.p, c.Fs32.p, c.B};
     waves = std::max(waves, (c.Jh + 15) / 16);
@@ -854,7 +874,7 @@ static void anal_multi(const std::vector<SphereCore *> &gs, const std::vector<do
   MultiAnal M{};
   for (size_t g = 0; g < gs.size(); g++) {
     SphereCore &c = *gs[g];
-    M.g[g] = AnalGroup{c.geo(), AnalTabs{c.A32.p, c.Abh.p, c.Abl.p, c.Ab3.p, nullptr}, c.Gb32.p, c.x0.p, c.xs.p, c.acc.p, dts[g], c.prm.diffusion,
+    M.g[g] = AnalGroup{c.geo(), AnalTabs{c.A32.p, c.Abh.p, c.Abl.p, c.Abf.p, nullptr}, c.Gb32.p, c.x0.p, c.xs.p, c.acc.p, dts[g], c.prm.diffusion,
                        c.B};
   }
   M.work = w.anal.p;

@@ -79,7 +79,8 @@ struct SphereCore {
   DevArray<int> d_mrow, d_soff, d_ke32, d_ko32;
   DevArray<float> A32, Ps32, Hs32;
   DevArray<double> A64;  // FP64 forward-transform operand of the FP64 MFMA variant (built on first use)
-  DevArray<__bf16> Abh, Abl, Sph, Spl, Shh, Shl, Ab3, Sp3, Sh3;  // BF16 parts 1, 2 (and 3 for BF16X6)
+  DevArray<__bf16> Abh, Abl, Sph, Spl, Shh, Shl;  // BF16 parts 1, 2 (BF16X3)
+  DevArray<float> Abf, Spf, Shf;                  // the same FP32 values in that layout, split in three on load (BF16X6)
   DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K
   DevArray<double2> x0, xs, acc;               // [b][3][S]
   DevArray<double2> Fs;                        // [b][J][M1][4]
