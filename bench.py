@@ -285,6 +285,11 @@ def ensemble_bench(lmw, args) -> dict:
     return out
 
 
+# sphere_adjustment.lms:18: a Gaussian low in a resting 5960 m layer, plus a zonal wind
+FIELDS_EXPR = ("5960.0 - 100.0 * pow(2.718281828459045, -((lon * cos(lat)) * (lon * cos(lat)) + (lat - 0.7853981633974483) * "
+               "(lat - 0.7853981633974483)) / 0.04)", "20.0 * cos(lat)", "0")
+
+
 def setup_bench(lmw, args):
     """The ensemble initial-condition path: set_state_grid of `members` T170 members from host grids, one member per
     call (the reference's shape) against the batched call (lmw_ensemble_set_state_grids), and the device-side
@@ -332,6 +337,13 @@ def setup_bench(lmw, args):
         pl.perturb(1.0, 100 + b, b)
     out["plane_perturb_1024x512_per_s"] = 64 / (time.perf_counter() - t0)
     del pl
+    # Sphere.fields (main.cpp:158-172) of every member from expressions: compiled once, evaluated on the device
+    ens = lmw.Ensemble([N] * members, lmw.SphereParams(A, OM, G), prec)
+    ens.fields(0, 2, *FIELDS_EXPR)
+    t0 = time.perf_counter()
+    ens.fields(0, members, *FIELDS_EXPR)
+    out["device_fields_expr_per_s"] = members / (time.perf_counter() - t0)
+    del ens
     if not args.no_cpu:
         from oracle import pyapi
         kind = "reference" if pyapi.available("ref") else "port"
@@ -347,6 +359,14 @@ def setup_bench(lmw, args):
         for _ in range(n):
             s.set_state_grid(h, u, v)
         wall = time.perf_counter() - t0
+        from oracle import expr as OX
+        if OX.ref_available():  # the reference's Sphere.fields: its compiler over the grid, then set_state_grid
+            lon, lat = OX.sphere_coords(s.gauss()[0], s.nlon)
+            LON, LAT = np.meshgrid(lon, lat)
+            t0 = time.perf_counter()
+            g = [OX.ref_eval(t, ["lon", "lat"], OX.sphere_params(A, OM, G), LON, LAT).reshape(LON.shape) for t in FIELDS_EXPR]
+            s.set_state_grid(*g)
+            out["cpu_reference_fields_expr_per_s"] = 1 / (time.perf_counter() - t0)
         out["cpu_baseline"] = {"value": n / wall, "unit": "member set-ups/s", "cores": 1, "kind": kind,
                                "sample": f"{n} set_state_grid calls of one T170 member, single thread, {wall:.2f} s"}
     return out

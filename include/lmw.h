@@ -189,6 +189,17 @@ lmw_status lmw_sphere_wave_crest(lmw_sphere *s, int m, double lat, double *out);
  * reference's operators do (cpp/main.cpp:174-291).  LMW_EINVAL "angle must be a finite number" for a non-finite
  * alpha. */
 lmw_status lmw_sphere_setup(lmw_sphere *s, int which, double alpha);
+/* expression grids (the Sphere.fields / orography / tracer operators, cpp/main.cpp:158-172, :212-228, :252-268,
+ * over the expression language of cpp/expr.hpp:16-166): NUL-terminated formulas in lon, lat (radians) and the
+ * planet's a, omega, g, compiled on the host and evaluated at every Gauss grid point on the device.
+ *   fields: h, u, v, then set_state_grid; orography: set_orography_grid; tracer: set_tracer_grid.
+ * LMW_ERUNTIME with the reference's texts: 'expression "<text>": <what> at column <c>' for a string that does not
+ * compile, "Sphere.fields: an expression is not finite on the grid" (h + u + v checked per point), "Sphere.orography:
+ * the expression is not finite on the grid", "Sphere.tracer: the expression is not finite on the grid"; the state is
+ * unchanged then. */
+lmw_status lmw_sphere_fields(lmw_sphere *s, const char *h, const char *u, const char *v);
+lmw_status lmw_sphere_orography(lmw_sphere *s, const char *hs);
+lmw_status lmw_sphere_tracer(lmw_sphere *s, const char *q);
 /* text output, "%.10g" per number (the Sphere.save_csv / save_spectrum / save_zonal_mean operators,
  * cpp/main.cpp:357-371, :328-335, :373-383): rows "lon lat h u v zeta q hs" in degrees with a blank line after each
  * latitude row; "n ke pe" for n = 1..T; "lat mean_h mean_u" per latitude.  LMW_ERUNTIME "<who>: cannot write
@@ -223,6 +234,11 @@ lmw_status lmw_ensemble_set_state_grids(lmw_ensemble *e, int member0, int count,
 /* lmw_sphere_setup for members member0 .. member0 + count - 1 in batched launches; LMW_SETUP_WILLIAMSON1 needs a
  * tracer, which ensemble members do not carry (LMW_EINVAL) */
 lmw_status lmw_ensemble_setup(lmw_ensemble *e, int member0, int count, int which, double alpha);
+/* lmw_sphere_fields / lmw_sphere_orography for members member0 .. member0 + count - 1: each truncation group's grid
+ * is evaluated once and analysed in batched launches; members given one orography share one device plane.  A
+ * non-finite value in a later group leaves the earlier groups set. */
+lmw_status lmw_ensemble_fields(lmw_ensemble *e, int member0, int count, const char *h, const char *u, const char *v);
+lmw_status lmw_ensemble_orography(lmw_ensemble *e, int member0, int count, const char *hs);
 lmw_status lmw_ensemble_set_orography_grid(lmw_ensemble *e, int member, const double *hs, size_t n);
 lmw_status lmw_ensemble_set_axis_tilt(lmw_ensemble *e, int member, double alpha);
 lmw_status lmw_ensemble_set_spectral(lmw_ensemble *e, int member, int field, const double *re_im, size_t n_complex);
@@ -314,6 +330,11 @@ lmw_status lmw_plane_reset(lmw_plane *p);
 lmw_status lmw_plane_perturb(lmw_plane *p, int member, double amplitude, uint64_t seed);
 /* set_h/u/v_from_grid: cpp/plane.hpp:57-59 / plane.cpp:312-318 and the accessors h/u/v(i, j) */
 lmw_status lmw_plane_set_field(lmw_plane *p, int member, int field, const double *g, size_t n);
+/* Plane.fields(h, u, v) (cpp/main.cpp:410-431): formulas in x, y and lx, ly, H, g, f0, beta, evaluated on the
+ * device: h = H + h(x, y) at the cell centres, u at the west faces, v at the interior south faces (the wall rows stay
+ * 0).  LMW_ERUNTIME with the expression compiler's texts, or "Plane.fields: the height|u|v expression is not
+ * finite" (state unchanged). */
+lmw_status lmw_plane_fields(lmw_plane *p, int member, const char *h, const char *u, const char *v);
 lmw_status lmw_plane_get_field(lmw_plane *p, int member, int field, double *out, size_t n);
 /* step(dt): cpp/plane.hpp:41 / plane.cpp:143-158 (all members) */
 lmw_status lmw_plane_step(lmw_plane *p, double dt);

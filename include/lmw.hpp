@@ -137,6 +137,10 @@ class Sphere {
   std::vector<double> grid_orography() const { return grid(LMW_GRID_OROGRAPHY); }
   // a Williamson et al. (1992) test case set up on the device (LMW_SETUP_*; the williamson1/2/5/6 operators)
   void setup(int which, double alpha = 0.0) { check(lmw_sphere_setup(h_, which, alpha)); }
+  // the expression operators Sphere.fields / orography / tracer (cpp/main.cpp:158-172,212-228,252-268)
+  void fields(const std::string &h, const std::string &u, const std::string &v) { check(lmw_sphere_fields(h_, h.c_str(), u.c_str(), v.c_str())); }
+  void orography(const std::string &hs) { check(lmw_sphere_orography(h_, hs.c_str())); }
+  void tracer(const std::string &q) { check(lmw_sphere_tracer(h_, q.c_str())); }
   // the text files of the Sphere.save_csv / save_spectrum / save_zonal_mean operators (cpp/main.cpp:328-383)
   void save_csv(const std::string &path) const { check(lmw_sphere_save_csv(h_, path.c_str())); }
   void save_spectrum(const std::string &path) const { check(lmw_sphere_save_spectrum(h_, path.c_str())); }
@@ -295,6 +299,8 @@ class Plane {
     const double uj = j < ny ? U(i % nx, j) : U(i % nx, ny - 1), ujm = j > 0 ? U(i % nx, j - 1) : U(i % nx, 0);
     return dvdx - (uj - ujm) / dy();
   }
+  // the Plane.fields operator (cpp/main.cpp:410-431): h = H + h(x, y), u, v from expressions
+  void fields(const std::string &h, const std::string &u, const std::string &v) { check(lmw_plane_fields(h_, 0, h.c_str(), u.c_str(), v.c_str())); }
   // the text files of the Plane.save_csv / save_row / save_column operators (cpp/main.cpp:464-493)
   void save_csv(const std::string &path) const { check(lmw_plane_save_csv(h_, 0, path.c_str())); }
   void save_row(int row, const std::string &path) const { check(lmw_plane_save_row(h_, 0, row, path.c_str())); }

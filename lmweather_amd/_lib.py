@@ -157,7 +157,18 @@ SIGNATURES = {
     "lmw_plane_save_csv": [_vp, C.c_int, C.c_char_p],
     "lmw_plane_save_row": [_vp, C.c_int, C.c_int, C.c_char_p],
     "lmw_plane_save_column": [_vp, C.c_int, C.c_int, C.c_char_p],
+    "lmw_sphere_fields": [_vp, C.c_char_p, C.c_char_p, C.c_char_p],
+    "lmw_sphere_orography": [_vp, C.c_char_p],
+    "lmw_sphere_tracer": [_vp, C.c_char_p],
+    "lmw_ensemble_fields": [_vp, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p],
+    "lmw_ensemble_orography": [_vp, C.c_int, C.c_int, C.c_char_p],
+    "lmw_plane_fields": [_vp, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p],
 }
+
+
+def text(s: str) -> bytes:
+    """An expression string as the bytes the library compiles (UTF-8, NUL-terminated by ctypes)."""
+    return s.encode("utf-8") if isinstance(s, str) else bytes(s)
 
 _lib = None
 

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblmw_hip.so")
 ARCH = os.environ.get("LMW_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sphere_core.hip", "sphere_abi.hip", "plane.hip", "plane_f32.hip", "precision.hip", "corrector.hip", "autotune.hip", "output.hip"]
+SOURCES = ["sphere_core.hip", "sphere_abi.hip", "plane.hip", "plane_f32.hip", "precision.hip", "corrector.hip", "autotune.hip", "output.hip", "expr.hip"]
 
 FLAGS = [
     "-O3",

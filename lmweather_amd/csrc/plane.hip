@@ -98,6 +98,16 @@ lmw_status lmw_plane_perturb(lmw_plane *p, int member, double amplitude, uint64_
     pl.core->perturb(member, amplitude, seed);
   });
 }
+lmw_status lmw_plane_fields(lmw_plane *p, int member, const char *h, const char *u, const char *v) {
+  return guard([&] {
+    lmw_plane &pl = P(p);
+    check_member(pl, member);
+    NEEDP(h);
+    NEEDP(u);
+    NEEDP(v);
+    pl.core->fields_expr(member, h, u, v);
+  });
+}
 lmw_status lmw_plane_set_field(lmw_plane *p, int member, int field, const double *g, size_t n) {
   return guard([&] {
     lmw_plane &pl = P(p);

@@ -21,6 +21,7 @@
 #include <memory>
 
 #include "common.h"
+#include "expr.h"
 
 namespace lmw {
 
@@ -647,6 +648,31 @@ template <class T> __global__ void k_widen(const T *__restrict__ src, double *__
   if (i < n) dst[i] = static_cast<double>(src[i]);
 }
 
+// Plane.fields (cpp/main.cpp:410-431) on the device: h = H + e0 at the cell centres, u = e1 at the west faces, v = e2
+// at the interior south faces (rows 1 .. ny-1; the wall rows 0 and ny are 0), written in FP64 into the staging
+// state io = [h][u][v].  xy = [nx] centre x, [nx] face x, [ny] centre y, [ny + 1] face y.  bad[f] is set where
+// field f is not finite.
+template <int D>
+__global__ __launch_bounds__(256) void k_plane_expr(ExprSet X, const double *__restrict__ xy, int nx, int ny, double depth,
+                                                    double *__restrict__ io, int *__restrict__ bad) {
+  const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+  if (i >= nx) return;
+  const double *xc = xy, *xu = xy + nx, *yc = xy + 2 * nx, *yv = xy + 2 * nx + ny;
+  const size_t nh = static_cast<size_t>(nx) * ny, at = static_cast<size_t>(j) * nx + i;
+  if (j < ny) {
+    const double h = depth + expr_eval<D>(X.code + X.start[0], X.len[0], xc[i], yc[j]);
+    const double u = expr_eval<D>(X.code + X.start[1], X.len[1], xu[i], yc[j]);
+    if (!isfinite(h)) bad[0] = 1;
+    if (!isfinite(u)) bad[1] = 1;
+    io[at] = h;
+    io[nh + at] = u;
+  }
+  double v = 0;
+  if (j >= 1 && j < ny) v = expr_eval<D>(X.code + X.start[2], X.len[2], xc[i], yv[j]);
+  if (!isfinite(v)) bad[2] = 1;
+  io[2 * nh + at] = v;
+}
+
 // The seeded noise of perturb (plane.cpp:34-57) on the device.  splitmix64 advances its state by a constant, so the
 // k-th draw of the stream is mix(st0 + k gamma) and every cell's twelve draws (12 c + 1 .. 12 c + 12) are computed
 // independently; the twelve terms are summed in draw order, as the sequential stream sums them, so the noise is bit
@@ -729,6 +755,8 @@ struct PlaneCoreBase {
   virtual void reset() = 0;
   // perturb (plane.cpp:34-57) of member b's h
   virtual void perturb(int b, double amplitude, uint64_t seed) = 0;
+  // Plane.fields (cpp/main.cpp:410-431) of member b from expressions in x, y and lx, ly, H, g, f0, beta
+  virtual void fields_expr(int b, const std::string &h, const std::string &u, const std::string &v) = 0;
   void sync() { LMW_HIP(hipStreamSynchronize(stream)); }
 };
 
@@ -790,6 +818,43 @@ template <class T> struct PlaneCore : PlaneCoreBase {
     const double rms = std::sqrt(ms / static_cast<double>(cells));
     const double scale = rms > 0 ? amplitude / rms : 0;
     hipLaunchKernelGGL(k_plane_add_noise<T>, dim3(blocks), dim3(256), 0, stream, x0.p + field_off(b, 0), n, mean, scale, cells);
+    LMW_LAUNCH_CHECK();
+    sync();
+  }
+  ExprDevice expr;
+  DevArray<double> xy;
+  void fields_expr(int b, const std::string &h, const std::string &u, const std::string &v) override {
+    const std::vector<std::pair<std::string, double>> par = {{"lx", prm.lx}, {"ly", prm.ly}, {"H", prm.depth},
+                                                             {"g", prm.gravity}, {"f0", prm.f0}, {"beta", prm.beta}};
+    const ExprProgram ph = compile_expr(h, "x", "y", par), pu = compile_expr(u, "x", "y", par), pv = compile_expr(v, "x", "y", par);
+    const ExprProgram *progs[3] = {&ph, &pu, &pv};
+    const ExprSet X = expr.load(progs, 3, stream);
+    // the coordinates as the reference forms them on the host (plane.hpp:26-31, main.cpp:418-422)
+    const int nx = prm.nx, ny = prm.ny;
+    const double dx = prm.lx / nx, dy = prm.ly / ny;
+    std::vector<double> c(2 * static_cast<size_t>(nx) + 2 * ny + 1);
+    for (int i = 0; i < nx; i++) {
+      c[i] = (i + 0.5) * dx;
+      c[nx + i] = i * dx;
+    }
+    for (int j = 0; j < ny; j++) c[2 * nx + j] = (j + 0.5) * dy;
+    for (int j = 0; j <= ny; j++) c[2 * nx + ny + j] = j * dy;
+    if (xy.n < c.size()) xy.alloc(c.size());
+    xy.upload(c.data(), c.size(), 0, stream);
+    const dim3 grd((nx + 255) / 256, ny + 1);
+    if (expr.depth <= kExprSmallDepth)
+      hipLaunchKernelGGL(k_plane_expr<kExprSmallDepth>, grd, dim3(256), 0, stream, X, xy.p, nx, ny, prm.depth, io.p, expr.bad.p);
+    else
+      hipLaunchKernelGGL(k_plane_expr<kExprMaxDepth>, grd, dim3(256), 0, stream, X, xy.p, nx, ny, prm.depth, io.p, expr.bad.p);
+    LMW_LAUNCH_CHECK();
+    int bad[3];
+    expr.bad.download(bad, 3, 0, stream);
+    sync();
+    if (bad[0]) runtime("Plane.fields: the height expression is not finite");
+    if (bad[1]) runtime("Plane.fields: the u expression is not finite");
+    if (bad[2]) runtime("Plane.fields: the v expression is not finite");
+    const size_t n = per();  // h, u, v of member b are contiguous, in io's order
+    hipLaunchKernelGGL(k_convert<T>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, io.p, x0.p + field_off(b, 0), n);
     LMW_LAUNCH_CHECK();
     sync();
   }

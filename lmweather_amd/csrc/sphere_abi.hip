@@ -486,6 +486,29 @@ lmw_status lmw_sphere_setup(lmw_sphere *s, int which, double alpha) {
     s->core->setup(0, 1, which, alpha);
   });
 }
+lmw_status lmw_sphere_fields(lmw_sphere *s, const char *h, const char *u, const char *v) {
+  return guard([&] {
+    NEED(s);
+    NEED(h);
+    NEED(u);
+    NEED(v);
+    s->core->fields_expr(0, 1, h, u, v);
+  });
+}
+lmw_status lmw_sphere_orography(lmw_sphere *s, const char *hs) {
+  return guard([&] {
+    NEED(s);
+    NEED(hs);
+    s->core->orography_expr(0, 1, hs);
+  });
+}
+lmw_status lmw_sphere_tracer(lmw_sphere *s, const char *q) {
+  return guard([&] {
+    NEED(s);
+    NEED(q);
+    s->core->tracer_expr(q);
+  });
+}
 lmw_status lmw_sphere_perturb(lmw_sphere *s, double amplitude, uint64_t seed, int max_wave) {
   return guard([&] {
     NEED(s);
@@ -785,6 +808,22 @@ lmw_status lmw_ensemble_setup(lmw_ensemble *e, int member0, int count, int which
     NEED(e);
     if (which == LMW_SETUP_WILLIAMSON1) invalid("a tracer needs a single-member sphere");
     member_runs(e, member0, count, false, [&](SphereCore &c, int b0, int nb, int) { c.setup(b0, nb, which, alpha); });
+  });
+}
+lmw_status lmw_ensemble_fields(lmw_ensemble *e, int member0, int count, const char *h, const char *u, const char *v) {
+  return guard([&] {
+    NEED(e);
+    NEED(h);
+    NEED(u);
+    NEED(v);
+    member_runs(e, member0, count, false, [&](SphereCore &c, int b0, int nb, int) { c.fields_expr(b0, nb, h, u, v); });
+  });
+}
+lmw_status lmw_ensemble_orography(lmw_ensemble *e, int member0, int count, const char *hs) {
+  return guard([&] {
+    NEED(e);
+    NEED(hs);
+    member_runs(e, member0, count, false, [&](SphereCore &c, int b0, int nb, int) { c.orography_expr(b0, nb, hs); });
   });
 }
 lmw_status lmw_ensemble_set_orography_grid(lmw_ensemble *e, int member, const double *hs, size_t n) {

@@ -17,6 +17,7 @@
 #include <memory>
 
 #include "common.h"
+#include "expr.h"
 
 namespace lmw {
 
@@ -116,6 +117,9 @@ struct SphereCore {
   void set_axis_tilt(int b, double alpha);
   void update_flags(int b);
   void set_orography(int b, const double *hs_host);
+  // members b0 .. b0 + nb - 1 get the orography hs_host (host bits, for the content key); a new plane is filled from
+  // the device copy dev_src when given, else uploaded
+  void set_orography_run(int b0, int nb, const double *hs_host, const double *dev_src);
   void point_orography(int b, int plane);
   int own_orography(int b);
   void set_state_grid(int b, const double *h, const double *u, const double *v);
@@ -125,6 +129,14 @@ struct SphereCore {
   void state_from_in3(int b0, int nb);
   void mark_reference_range(int b0, int nb);
   void tracer_from_in3();
+  // expression grids (expr.hip): Sphere.fields / orography / tracer of cpp/main.cpp:158-172,212-228,252-268
+  ExprDevice expr;
+  DevArray<double> lonlat;  // [K] longitudes, [J] latitudes
+  void upload_lonlat();
+  bool eval_exprs(const ExprProgram *const progs[], int count, int nb);
+  void fields_expr(int b0, int nb, const std::string &h, const std::string &u, const std::string &v);
+  void orography_expr(int b0, int nb, const std::string &text);
+  void tracer_expr(const std::string &text);
   void set_spectral_packed(int b, int field, const double2 *packed);  // host packed triangle
   void get_spectral_packed(int b, int field, double2 *packed);
   void mark_reference(int b);

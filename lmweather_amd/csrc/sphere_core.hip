@@ -966,7 +966,9 @@ int SphereCore::own_orography(int b) {
 
 // set_orography_grid (sphere.cpp:105-109): member b shares a plane that already holds the same bits (checked on the
 // device copy), else gets one of its own
-void SphereCore::set_orography(int b, const double *hsh) {
+void SphereCore::set_orography(int b, const double *hsh) { set_orography_run(b, 1, hsh, nullptr); }
+
+void SphereCore::set_orography_run(int b0, int nb, const double *hsh, const double *dev_src) {
   const size_t plane = static_cast<size_t>(J) * K;
   uint64_t key = 1469598103934665603ull;  // FNV-1a over the 64-bit words
   for (size_t i = 0; i < plane; i++) {
@@ -983,16 +985,20 @@ void SphereCore::set_orography(int b, const double *hsh) {
     sync();
     if (std::memcmp(tmp.data(), hsh, plane * sizeof(double)) == 0) found = p;
   }
-  if (found >= 0) {
-    point_orography(b, found);
-  } else {
-    const int p = own_orography(b);
-    hs.upload(hsh, plane, static_cast<size_t>(p) * plane, stream);
-    hs_kind[p] = 1;
-    hs_key[p] = key;
+  if (found < 0) {
+    found = own_orography(b0);
+    if (dev_src)
+      LMW_HIP(hipMemcpyAsync(hs.p + static_cast<size_t>(found) * plane, dev_src, plane * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    else
+      hs.upload(hsh, plane, static_cast<size_t>(found) * plane, stream);
+    hs_kind[found] = 1;
+    hs_key[found] = key;
   }
-  has_orography[b] = 1;
-  update_flags(b);
+  for (int b = b0; b < b0 + nb; b++) {
+    point_orography(b, found);
+    has_orography[b] = 1;
+    update_flags(b);
+  }
 }
 
 template <int MB>

@@ -57,6 +57,11 @@ class Plane:
     def perturb(self, amplitude: float, seed: int, member: int = 0) -> None:
         L.call("lmw_plane_perturb", self._h, member, amplitude, seed)
 
+    def fields(self, h: str, u: str, v: str, member: int = 0) -> None:
+        """The Plane.fields operator (main.cpp:410-431): h = H + h(x, y) at the cell centres, u at the west faces, v
+        at the interior south faces, from expressions in x, y and lx, ly, H, g, f0, beta, evaluated on the device."""
+        L.call("lmw_plane_fields", self._h, member, L.text(h), L.text(u), L.text(v))
+
     def _rows(self, field):
         return self.ny + 1 if field == L.PLANE_V else self.ny
 

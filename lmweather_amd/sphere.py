@@ -160,6 +160,19 @@ class Sphere:
         williamson5(), williamson6() operators of main.cpp:174-291), evaluated and analysed on the device."""
         L.call("lmw_sphere_setup", self._h, int(which), float(alpha))
 
+    def fields(self, h: str, u: str, v: str) -> None:
+        """The Sphere.fields operator (main.cpp:158-172): h, u, v from expressions in lon, lat and a, omega, g
+        (cpp/expr.hpp), evaluated at the Gauss grid on the device, then set_state_grid."""
+        L.call("lmw_sphere_fields", self._h, L.text(h), L.text(u), L.text(v))
+
+    def orography(self, hs: str) -> None:
+        """The Sphere.orography operator (main.cpp:212-228): set_orography_grid of an expression in lon, lat."""
+        L.call("lmw_sphere_orography", self._h, L.text(hs))
+
+    def tracer(self, q: str) -> None:
+        """The Sphere.tracer operator (main.cpp:252-268): set_tracer_grid of an expression in lon, lat."""
+        L.call("lmw_sphere_tracer", self._h, L.text(q))
+
     def add_mode(self, n: int, m: int, amplitude: float, mean_depth: float) -> None:
         L.call("lmw_sphere_add_mode", self._h, n, m, amplitude, mean_depth)
 
@@ -371,6 +384,14 @@ class Ensemble:
     def setup(self, member0: int, count: int, which: int, alpha: float = 0.0) -> None:
         """Sphere.setup for members member0 .. member0 + count - 1 (test case 1 needs a tracer: single spheres only)."""
         L.call("lmw_ensemble_setup", self._h, int(member0), int(count), int(which), float(alpha))
+
+    def fields(self, member0: int, count: int, h: str, u: str, v: str) -> None:
+        """Sphere.fields for members member0 .. member0 + count - 1 (one grid per truncation group)."""
+        L.call("lmw_ensemble_fields", self._h, int(member0), int(count), L.text(h), L.text(u), L.text(v))
+
+    def orography(self, member0: int, count: int, hs: str) -> None:
+        """Sphere.orography for members member0 .. member0 + count - 1 (one shared plane per truncation group)."""
+        L.call("lmw_ensemble_orography", self._h, int(member0), int(count), L.text(hs))
 
     def set_orography_grid(self, member, hs) -> None:
         hs = L.as_f64(hs).ravel()

@@ -11,6 +11,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -18,6 +19,7 @@
 #include "plane.hpp"
 #include "sphere.hpp"
 #undef private
+#include "expr.hpp"
 
 using lmweather::cplx;
 using lmweather::Plane;
@@ -40,6 +42,19 @@ std::vector<cplx> getc(const double *p, size_t n) { return std::vector<cplx>(rei
 
 extern "C" {
 const char *ref_last_error(void) { return g_err.c_str(); }
+// the reference's expression compiler (cpp/expr.hpp) over two variables and named constants: out[i] = e(x0[i], x1[i])
+int ref_expr_eval(const char *text, const char *v0, const char *v1, int np, const char *const *pn, const double *pv, const double *x0,
+                  const double *x1, size_t n, double *out) {
+  return guard([&] {
+    std::map<std::string, double> par;
+    for (int i = 0; i < np; i++) par[pn[i]] = pv[i];
+    lmnum::Expr e(text, {v0, v1}, par);
+    for (size_t i = 0; i < n; i++) {
+      const double x[2] = {x0[i], x1[i]};
+      out[i] = e.eval(x);
+    }
+  });
+}
 
 int ref_fft(double *re_im, size_t n, int sign) {
   return guard([&] {

@@ -108,6 +108,16 @@ int main() {
     } catch (const std::invalid_argument &) { threw = true; }
     EXPECT(threw);
   }
+  {  // expression operators: solid-body rotation's kinetic energy per unit depth u0^2/3 (tests/sphere_spectrum.lms:13-15)
+    lmw::Sphere s(21, lmw::SphereParams{});
+    s.fields("2000.0", "15.0 * cos(lat)", "0");
+    CLOSE(s.kinetic_energy() / 2000.0, 75.0, 1e-7);
+    bool threw = false;
+    try {
+      s.fields("2 +* 3", "0", "0");
+    } catch (const std::runtime_error &e) { threw = std::string(e.what()) == "expression \"2 +* 3\": unexpected '*' at column 4"; }
+    EXPECT(threw);
+  }
   {  // plane: linear standing gravity wave after half a period, mass, bad parameters
     lmw::PlaneParams q;
     q.nx = 100;
