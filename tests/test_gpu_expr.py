@@ -1,7 +1,8 @@
 """GPU expression grids, through the C ABI: the Sphere.fields / orography / tracer and Plane.fields operators
 (cpp/main.cpp:158-172,212-228,252-268,410-431) compiled on the host and evaluated on the device, against the oracle's
 restatement of the reference's compiler (oracle/expr.py) feeding the oracle model, plus the reference's own script
-tests that use them (tests/sphere_spectrum.lms, tests/sphere_orography.lms, test.sh:179-215).
+tests that use them (tests/sphere_spectrum.lms, sphere_orography.lms, plane_rossby.lms, plane_adjustment.lms,
+test.sh:179-215).
 """
 import numpy as np
 import pytest
@@ -200,3 +201,50 @@ def test_ensemble_fields_and_orography(lmw):
         for f in range(3):
             assert max_rel(e.get_spectral(i, f), s.get_spectral(f), scale=np.max(np.abs(s.get_spectral(0 if f == 1 else f)))) <= 1e-14, (i, f)
         assert e.diag(i, 3) == pytest.approx(s.potential_energy(), rel=1e-14)
+
+
+ROSSBY = ("f0 * 100000.0 * cos(2 * pi * x / lx) * sin(pi * y / ly) / g", "-100000.0 * pi / ly * cos(2 * pi * x / lx) * cos(pi * y / ly)",
+          "-100000.0 * 2 * pi / lx * sin(2 * pi * x / lx) * sin(pi * y / ly)")
+
+
+def test_plane_script_known_answers(lmw):
+    """plane_rossby.lms (the beta drift of a Rossby wave, none on an f-plane, twice as fast at 2 beta) and the first
+    part of plane_adjustment.lms (deformation radius, PV and mass through five inertial periods), on the device."""
+    lx, ly, f0, beta = 4.0e6, 2.0e6, 1e-4, 2e-11
+
+    def plane(b):
+        p = lmw.Plane(lmw.PlaneParams(60, 30, lx, ly, depth=1000.0, f0=f0, beta=b, linear=True))
+        p.fields(*ROSSBY)
+        return p
+
+    k, l = 2 * np.pi / lx, np.pi / ly
+    m = plane(beta)
+    ld = m.rossby_radius()
+    cp = -beta / (k * k + l * l + 1.0 / (ld * ld))
+    assert abs(cp - -3.359) <= 0.002
+    c0, amp0 = m.zonal_crest(1, 15), m.zonal_amplitude(1, 15)
+    t = 20.0 * 86400.0
+    m.run(t, 0.0)
+    diff = m.zonal_crest(1, 15) - c0 - cp * t
+    assert abs(diff - np.floor(diff / lx + 0.5) * lx) <= 60000.0
+    assert abs(m.zonal_amplitude(1, 15) / amp0 - 1.0) <= 0.1
+    f = plane(0.0)
+    fc0 = f.zonal_crest(1, 15)
+    f.run(t, 0.0)
+    moved = f.zonal_crest(1, 15) - fc0
+    assert abs(moved - np.floor(moved / lx + 0.5) * lx) < 0.02 * lx
+    shifts = []
+    for b in (2 * beta, beta):
+        p = plane(b)
+        c = p.zonal_crest(1, 15)
+        p.run(2.0 * 86400.0, 0.0)
+        s = p.zonal_crest(1, 15) - c
+        shifts.append(s - np.floor(s / lx + 0.5) * lx)
+    assert abs(shifts[0] / shifts[1] - 2.0) <= 0.15
+    a = lmw.Plane(lmw.PlaneParams(8, 200, 40000.0, 4.0e6, depth=1000.0, f0=f0, beta=0.0, linear=True))
+    a.fields("exp(-(y - ly / 2) * (y - ly / 2) / 160000000000.0)", "0", "0")
+    assert abs(a.rossby_radius() - 990515.0) <= 100.0
+    q0, mean0 = a.potential_vorticity_anomaly_rms(), a.mean_depth()
+    a.run(5.0 * 2.0 * np.pi / f0, 0.0)
+    assert abs(a.potential_vorticity_anomaly_rms() / q0 - 1.0) <= 0.001
+    assert abs(a.mean_depth() - mean0) <= 1e-11
