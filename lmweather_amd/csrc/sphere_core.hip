@@ -87,6 +87,23 @@ template <class TS, class C>
 __device__ inline void store_operands(TS *Gb, size_t bm, int Jh, int jh, const C Xn[4], const C Xs[4], const C Yn[4], const C Ys[4]) {
   TS *o = Gb + bm * 16 * Jh + static_cast<size_t>(jh) * 4;
   const size_t part = static_cast<size_t>(Jh) * 4;
+  if constexpr (sizeof(TS) == 8) {  // FP32 records: 32 bytes, 32-byte aligned, written as two 16-byte stores each
+    TS v[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      v[0][i] = to_c<TS>(cadd(Xn[i], Xs[i]));
+      v[1][i] = to_c<TS>(csub(Yn[i], Ys[i]));
+      v[2][i] = to_c<TS>(csub(Xn[i], Xs[i]));
+      v[3][i] = to_c<TS>(cadd(Yn[i], Ys[i]));
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+      float4 *o4 = reinterpret_cast<float4 *>(o + p * part);
+      o4[0] = make_float4(v[p][0].x, v[p][0].y, v[p][1].x, v[p][1].y);
+      o4[1] = make_float4(v[p][2].x, v[p][2].y, v[p][3].x, v[p][3].y);
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     o[i] = to_c<TS>(cadd(Xn[i], Xs[i]));             // p 0, X
