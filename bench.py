@@ -6,9 +6,13 @@ a = 6.371e6 m, dt = 150 s) -- the T170 share of the 256-member uneven ensemble -
 relative noise (seed 1000 + member).  Inputs are resident in HBM before the timed region.
 
 value = whole-job member-days/s of the batch with the Legendre contractions in ``--precision`` (default BF16x6:
-FP32 operands split into three BF16 parts, six MFMA products, FP32 accumulation); the same batch in FP64 and one FP64 member ("plain") are timed beside it.  The accuracy of the mixed
-precision is the rel-L2 of h, u, v against FP64 after ``--check-days`` simulated days of one member, next to an FP64
-twin started from a 1e-7 relative vorticity perturbation (the dynamics' own error growth).  The roofline is that of
+FP32 operands split into three BF16 parts, six MFMA products, FP32 accumulation; the FFTs and grid products between
+the two contractions run in FP32 in the mixed modes); the same batch in FP64 and one FP64 member ("plain") are timed
+beside it.  The accuracy of the mixed precision is measured on the timed batch itself: every member's rel-L2 of h, u,
+v against the same member of the FP64 batch after ``--check-days`` simulated days (worst and mean over the members),
+next to an FP64 twin batch started from a further 1e-7 relative vorticity perturbation (the dynamics' own error
+growth).  Other workloads: ``--workload ensemble`` (C3), ``plane`` (C4), ``setup``, ``c1`` (one T42 member, the
+reference's shape) and ``accuracy`` (C2 single instance in every mode to 1 and ``--check-days`` days).  The roofline is that of
 the dominant kernel, measured with HIP events around every launch on the library's stream in a second pass of as
 many steps right after the timed region.  The reference CPU code
 (oracle/_ref, one core) is timed on a bounded sample of the same workload.
@@ -31,8 +35,10 @@ N, DT = 170, 150.0
 PEAKS = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15, "bf16x6": 2.5e15}
 MFMA_PRODUCTS = {"fp64": 1.0, "fp32": 1.0, "bf16x3": 3.0, "bf16x6": 6.0}  # MFMA products per multiply-add
 PREC_NAMES = ("fp64", "fp32", "bf16x3", "bf16x6")
-DTYPES = {"fp64": "f64", "fp32": "f32 Legendre, f64 state/FFT", "bf16x3": "bf16x3 Legendre (f32 acc), f64 state/FFT",
-          "bf16x6": "bf16x6 Legendre (f32 acc), f64 state/FFT"}
+DTYPES = {"fp64": "f64",
+          "fp32": "f32 Legendre and f32 FFT/grid products; f64 state/RK4/tables",
+          "bf16x3": "bf16x3 Legendre (f32 acc) and f32 FFT/grid products; f64 state/RK4/tables",
+          "bf16x6": "bf16x6 Legendre (f32 acc) and f32 FFT/grid products; f64 state/RK4/tables"}
 HBM_PEAK = 8.0e12
 
 
@@ -95,34 +101,103 @@ def timed_batch(ens, steps, warmup):
     return wall, ms
 
 
-def cpu_baseline(state, steps):
-    """The reference's own code (oracle/_ref, else the oracle restatement) on one host core: `steps` RK4 steps of
-    one T170 member from the same initial state."""
+def _cpu_member(args):
+    """One reference member on one core (run in a worker process): wall seconds of `steps` RK4 steps."""
+    state, steps, kind = args
     from oracle import pyapi
-    kind = "reference" if pyapi.available("ref") else "port"
     h, u, v, hs = state
-    s = pyapi.OracleSphere(N, A, OM, G, kind="ref" if kind == "reference" else "oracle")
+    s = pyapi.OracleSphere(N, A, OM, G, kind=kind)
     s.set_orography_grid(hs)
     s.set_state_grid(h, u, v)
     s.step(DT)  # warm caches
     t0 = time.perf_counter()
     for _ in range(steps):
         s.step(DT)
-    wall = time.perf_counter() - t0
-    return {"value": DT * steps / 86400.0 / wall, "unit": "member-days/s", "cores": 1, "kind": kind,
-            "sample": f"{steps} RK4 steps (dt {DT:g} s) of one T170 member, single thread, {wall:.2f} s",
-            "ms_per_step": 1e3 * wall / steps}
+    return time.perf_counter() - t0
 
 
-def accuracy(lmw, state, prec, days):
-    """rel-L2 of h, u, v of one member in `prec` and of an FP64 twin (1e-7 relative vorticity perturbation) against
-    FP64 after `days`, plus the energy and mass drift of each."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    """Cores this process may use, at most 16 (the share of one GPU's host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(state, steps):
+    """The reference's own code (oracle/_ref, else the oracle restatement) on the host cores: `steps` RK4 steps of
+    one T170 member on one core, then P independent members on P cores at once, one worker process each (the
+    reference is single-threaded; an ensemble spreads over cores).  value = the P-core ensemble rate."""
+    import multiprocessing as mp
+    from oracle import pyapi
+    kind = "reference" if pyapi.available("ref") else "port"
+    lib = "ref" if kind == "reference" else "oracle"
+    wall1 = _cpu_member((state, steps, lib))
+    one = DT * steps / 86400.0 / wall1
+    P = host_cores()
+    psteps = max(8, steps // 4)
+    ctx = mp.get_context("spawn")  # fresh interpreters: nothing of this process's GPU context is inherited
+    t0 = time.perf_counter()
+    with ctx.Pool(P) as pool:
+        walls = pool.map(_cpu_member, [(state, psteps, lib)] * P)
+    wall_p = time.perf_counter() - t0
+    agg = P * DT * psteps / 86400.0 / max(walls)
+    return {"value": agg, "unit": "member-days/s", "cores": P, "kind": kind, "cpu": cpu_model(),
+            "sample": f"{P} independent T170 members, {psteps} RK4 steps (dt {DT:g} s) each, one per core in "
+                      f"{P} worker processes, slowest {max(walls):.2f} s ({wall_p:.2f} s with process start-up)",
+            "single_core": {"value": one, "cores": 1, "ms_per_step": 1e3 * wall1 / steps,
+                            "sample": f"{steps} RK4 steps of one T170 member, single thread, {wall1:.2f} s"}}
+
+
+def batch_drifts(lmw, ens, members):
+    """(energy, mean height) of every member: the start of the conservation drifts."""
+    return [(ens.diag(i, lmw._lib.DIAG_ENERGY), ens.diag(i, lmw._lib.DIAG_MEAN_HEIGHT)) for i in range(members)]
+
+
+def batch_accuracy(lmw, ens, e64, twin, done, members, days, start):
+    """Accuracy of the timed batch itself: every member of the mixed batch `ens` and of the FP64 twin batch `twin`
+    (the FP64 batch with a further 1e-7 relative vorticity perturbation: the dynamics' own error growth) against the
+    same member of the FP64 batch `e64`, all advanced to `days` simulated days (done[k] = steps batch k has taken in
+    the timing passes).  Worst and mean over the members of rel-L2 of h, u, v; worst energy and mass drift."""
+    n = int(round(days * 86400 / DT))
+    for e, d in zip((ens, e64, twin), done):
+        if n > d:
+            e.steps([DT], [n - d])
+    out = {"days": days, "steps": n, "members": members, "measured_on": "the timed batch (all members)"}
+    for name, e in (("mixed", ens), ("twin", twin)):
+        for f, fld in enumerate("huv"):
+            errs = np.array([np.linalg.norm(e.grid(i, f) - e64.grid(i, f)) / np.linalg.norm(e64.grid(i, f))
+                             for i in range(members)])
+            out[f"{name}_rel_l2_{fld}_worst"] = float(errs.max())
+            out[f"{name}_rel_l2_{fld}_mean"] = float(errs.mean())
+    for name, e, k in (("fp64", e64, 1), ("mixed", ens, 0)):
+        de = [abs(e.diag(i, lmw._lib.DIAG_ENERGY) / start[k][i][0] - 1.0) for i in range(members)]
+        dm = [abs(e.diag(i, lmw._lib.DIAG_MEAN_HEIGHT) / start[k][i][1] - 1.0) for i in range(members)]
+        out[f"{name}_energy_drift_worst"] = float(max(de))
+        out[f"{name}_mass_drift_worst"] = float(max(dm))
+    return out
+
+
+def single_accuracy(lmw, state, days_list):
+    """Config C2 as a single instance: one T170 member (the C2 state over the mountains, no perturbation) stepped in
+    FP64, FP32, BF16x3 and BF16x6, and an FP64 twin from a 1e-7 relative vorticity perturbation, to each of
+    `days_list`; rel-L2 of h, u, v against FP64 and the energy and mass drift of each."""
     from lmweather_amd.initial import perturb_vorticity
     h, u, v, hs = state
-    n = int(round(days * 86400 / DT))
-    out = {}
     runs = {}
-    for name, pr, twin in (("fp64", lmw.FP64, False), ("mixed", prec, False), ("twin", lmw.FP64, True)):
+    for name, pr, twin in (("fp64", lmw.FP64, False), ("fp32", lmw.FP32, False), ("bf16x3", lmw.BF16X3, False),
+                           ("bf16x6", lmw.BF16X6, False), ("fp64_twin", lmw.FP64, True)):
         s = lmw.Sphere(N, lmw.SphereParams(A, OM, G))
         s.set_orography_grid(hs)
         s.set_state_grid(h, u, v)
@@ -130,17 +205,98 @@ def accuracy(lmw, state, prec, days):
             s.set_spectral(0, perturb_vorticity(s.get_spectral(0), 1e-7, 77))
         if pr != lmw.FP64:
             s.set_precision(pr)
-        e0, m0 = s.energy(), s.mean_height()
-        s.steps(DT, n)
-        runs[name] = (s.grid_height(), s.grid_u(), s.grid_v())
-        out[f"{name}_energy_drift"] = (s.energy() - e0) / e0
-        out[f"{name}_mass_drift"] = (s.mean_height() - m0) / m0
-    for name in ("mixed", "twin"):
-        for i, f in enumerate("huv"):
-            a, b = runs[name][i], runs["fp64"][i]
-            out[f"{name}_rel_l2_{f}"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    out["days"] = days
+        runs[name] = (s, s.energy(), s.mean_height())
+    out, done = {}, 0
+    for days in days_list:
+        n = int(round(days * 86400 / DT))
+        t0 = time.perf_counter()
+        for s, _, _ in runs.values():
+            s.steps(DT, n - done)
+        done = n
+        print(f"accuracy: day {days:g} reached in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        ref = runs["fp64"][0]
+        rg = [ref.grid_height(), ref.grid_u(), ref.grid_v()]
+        rec = {}
+        for name, (s, e0, m0) in runs.items():
+            r = {"energy_drift": s.energy() / e0 - 1.0, "mass_drift": s.mean_height() / m0 - 1.0}
+            if name != "fp64":
+                for f, g in zip("huv", (s.grid_height(), s.grid_u(), s.grid_v())):
+                    r[f"rel_l2_{f}"] = float(np.linalg.norm(g - rg["huv".index(f)]) / np.linalg.norm(rg["huv".index(f)]))
+            rec[name] = r
+        out[f"day_{days:g}"] = rec
     return out
+
+
+def c1_cpu_reference(state, seconds, dt):
+    """CPU baseline leg of config C1: the reference's own code (oracle/_ref, else the oracle) on one core runs the
+    same run(seconds, dt) from the same grids; returns its timing, its final height grid (the checker of the GPU
+    result) and its drifts."""
+    from oracle import pyapi
+    kind = "reference" if pyapi.available("ref") else "port"
+    h, u, v = state
+    s = pyapi.OracleSphere(42, A, OM, G, kind="ref" if kind == "reference" else "oracle")
+    s.set_state_grid(h, u, v)
+    e0, q0, m0 = s.energy(), s.potential_enstrophy(), s.mean_height()
+    t0 = time.perf_counter()
+    s.run(seconds, dt)
+    wall = time.perf_counter() - t0
+    steps = int(round(seconds / dt))
+    return {"value": seconds / 86400.0 / wall, "unit": "member-days/s", "cores": 1, "kind": kind, "cpu": cpu_model(),
+            "sample": f"the whole run: {steps} RK4 steps (dt {dt:g} s) of one T42 member, single thread, {wall:.2f} s",
+            "ms_per_step": 1e3 * wall / steps}, s.grid_height(), {
+        "energy_drift": s.energy() / e0 - 1.0, "enstrophy_drift": s.potential_enstrophy() / q0 - 1.0,
+        "mass_drift": s.mean_height() / m0 - 1.0}
+
+
+def c1_bench(lmw, args) -> dict:
+    """Config C1: one T42 member (128 x 64 Gaussian grid, a = 6.371e6 m), the balanced random flow n <= 21 (seed 1,
+    max |u| 40 m/s, depth 8000 m), FP64, run(5 days, 600 s) = 720 RK4 steps with the reference's checks every 64
+    steps -- the reference's own shape, a single instance.  value = member-days/s of the GPU run; beside it the
+    reference on one core over the same run, the GPU's rel-L2(h) against it at day 5 and the drifts of both."""
+    from lmweather_amd.initial import random_balanced_state
+    seconds, dt = 5 * 86400.0, 600.0
+    s = lmw.Sphere(42, lmw.SphereParams(A, OM, G))
+    h, u, v = random_balanced_state(s, 21, 1)
+    s.set_state_grid(h, u, v)
+    s.steps(dt, max(args.warmup, 1))  # graph capture and first launches
+    s.set_state_grid(h, u, v)
+    e0, q0, m0 = s.energy(), s.potential_enstrophy(), s.mean_height()
+    t0 = time.perf_counter()
+    s.run(seconds, dt)
+    wall = time.perf_counter() - t0
+    steps = int(round(seconds / dt))
+    out = {
+        "metric": "member-days/s, single T42 instance (FP64)", "value": seconds / 86400.0 / wall,
+        "unit": "member-days/s", "n_gpus": 1, "steps": steps, "warmup": max(args.warmup, 1),
+        "ms_per_step": 1e3 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "T42 single instance (128x64 Gaussian grid), balanced random flow n<=21 seed 1, "
+                               "run(5 days, dt 600 s) with the checks every 64 steps", "truncation": 42, "dt_s": dt},
+        "drift": {"energy": s.energy() / e0 - 1.0, "enstrophy": s.potential_enstrophy() / q0 - 1.0,
+                  "mass": s.mean_height() / m0 - 1.0},
+    }
+    if not args.no_cpu:
+        print("cpu baseline: the reference's 720 steps", file=sys.stderr, flush=True)
+        out["cpu_baseline"], href, out["cpu_drift"] = c1_cpu_reference((h, u, v), seconds, dt)
+        hg = s.grid_height()
+        out["rel_l2_h_vs_reference_day5"] = float(np.linalg.norm(hg - href) / np.linalg.norm(href))
+        out["speedup_vs_cpu_core"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def accuracy_bench(lmw, args) -> dict:
+    """Config C2 accuracy: one T170 member in every precision mode and an FP64 twin, to 1 and 10 days."""
+    state = initial_state(lmw)
+    days = [1.0, args.check_days] if args.check_days > 1.0 else [args.check_days]
+    t0 = time.perf_counter()
+    res = single_accuracy(lmw, state, days)
+    return {"metric": "rel-L2 of h, u, v against FP64, single T170 instance", "value": res[f"day_{days[-1]:g}"]["bf16x6"]["rel_l2_h"],
+            "unit": "rel-L2(h) of BF16x6 at the last day", "n_gpus": 1, "steps": int(round(days[-1] * 86400 / DT)),
+            "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64 reference; f32, bf16x3, bf16x6 Legendre with f32 FFT/grid products", "data": "synthetic",
+            "config": {"workload": "T170 single instance (512x256 Gaussian grid, dt 150 s), balanced random flow n<=85 "
+                                   "over 8 Gaussian mountains (C2)", "days": days},
+            "wall_s": time.perf_counter() - t0, "accuracy": res}
 
 
 def plane_jet_state(nx, ny, lx, ly, g, H, f0, beta, U=1.0, Lw=1.0e5):
@@ -374,7 +530,7 @@ def setup_bench(lmw, args):
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup"])
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup", "c1", "accuracy"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=4)
@@ -402,6 +558,12 @@ def main() -> int:
     if args.workload == "setup":
         print(json.dumps(setup_bench(lmw, args)))
         return 0
+    if args.workload == "c1":
+        print(json.dumps(c1_bench(lmw, args)))
+        return 0
+    if args.workload == "accuracy":
+        print(json.dumps(accuracy_bench(lmw, args)))
+        return 0
     prec = PREC_NAMES.index(args.precision)
     mixed = prec != lmw.FP64
     name, cus, _ = lmw.device_info()
@@ -410,8 +572,10 @@ def main() -> int:
     ens = build_batch(lmw, args.members, prec, state)
     _, J, K = ens.dims(0)
     S = (N + 1) * (N + 2) // 2
+    start = [batch_drifts(lmw, ens, args.members)] if mixed and not args.only_batch else None
     wall, ms = timed_batch(ens, args.steps, args.warmup)
-    del ens
+    if start is None:
+        del ens
     value = args.members * DT * args.steps / 86400.0 / wall
     print(f"{args.precision}: timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
 
@@ -463,9 +627,9 @@ def main() -> int:
     if not args.only_batch:
         if mixed:
             e64 = build_batch(lmw, args.members, lmw.FP64, state)
+            start.append(batch_drifts(lmw, e64, args.members))
             n64 = max(args.steps // 2, 4)
             w64, _ = timed_batch(e64, n64, args.warmup)
-            del e64
             out["fp64_batched_member_days_per_s"] = args.members * DT * n64 / 86400.0 / w64
         p = lmw.SphereParams(A, OM, G)
         one = lmw.Sphere(N, p)
@@ -480,8 +644,14 @@ def main() -> int:
         if "fp64_batched_member_days_per_s" in out:
             out["speedup_vs_fp64_batched"] = value / out["fp64_batched_member_days_per_s"]
         if mixed:
-            print(f"accuracy: {args.check_days} days", file=sys.stderr, flush=True)
-            out["accuracy_vs_fp64"] = accuracy(lmw, state, prec, args.check_days)
+            print(f"accuracy: {args.check_days} days of the timed batch against the FP64 batch", file=sys.stderr, flush=True)
+            from lmweather_amd.initial import perturb_vorticity
+            twin = build_batch(lmw, args.members, lmw.FP64, state)
+            for i in range(args.members):
+                twin.set_spectral(i, 0, perturb_vorticity(twin.get_spectral(i, 0), 1e-7, 77 + i))
+            done = (args.warmup + 2 * args.steps, args.warmup + 2 * n64, 0)
+            out["accuracy_vs_fp64"] = batch_accuracy(lmw, ens, e64, twin, done, args.members, args.check_days, start)
+            del ens, e64, twin
         if not args.no_cpu:
             print(f"cpu baseline: {args.cpu_steps} reference steps", file=sys.stderr, flush=True)
             out["cpu_baseline"] = cpu_baseline(state, args.cpu_steps)

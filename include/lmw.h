@@ -33,8 +33,11 @@ typedef int lmw_status;
 #define LMW_ERUNTIME 2
 #define LMW_EDEVICE 3
 
-/* arithmetic of the Legendre contractions; everything else (FFTs, grid products, RK4, tables, quadrature,
- * conservation sums) is FP64 in every mode */
+/* arithmetic of the time step's transforms.  LMW_FP64: everything in FP64.  The three mixed modes change the two
+ * Legendre contractions (below) AND the grid-point kernel between them: its longitude FFTs, the nonlinear products
+ * and the Bernoulli function run in FP32 on FP32 Fourier rows.  The state, the RK4 update, the Legendre tables
+ * (built in FP64, then rounded or split), the Gauss nodes and weights, set-up, diagnostics and the conservation
+ * sums stay FP64 in every mode. */
 #define LMW_FP64 0
 #define LMW_FP32 1   /* FP32 tables and operands, FP32 accumulation */
 #define LMW_BF16X3 2 /* split BF16 hi+lo operands, three MFMA products, FP32 accumulation */
