@@ -63,13 +63,13 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
 
 
 def build_cpp_tests() -> str:
-    """Compile tests/core_gpu.cpp (the reference's native checks restated on the C++ facade include/lmw.hpp) into
-    tests/bin/core_gpu, linked against liblmw_hip.so."""
+    """Compile tests/facade_checks.cpp (closed-form checks of the C++ facade include/lmw.hpp) into
+    tests/bin/facade_checks, linked against liblmw_hip.so."""
     out_dir = os.path.join(ROOT, "tests", "bin")
     os.makedirs(out_dir, exist_ok=True)
-    exe = os.path.join(out_dir, "core_gpu")
+    exe = os.path.join(out_dir, "facade_checks")
     cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
-           os.path.join(ROOT, "tests", "core_gpu.cpp"), "-L", HERE, "-llmw_hip", "-Wl,-rpath,$ORIGIN/../../lmweather_amd", "-o", exe]
+           os.path.join(ROOT, "tests", "facade_checks.cpp"), "-L", HERE, "-llmw_hip", "-Wl,-rpath,$ORIGIN/../../lmweather_amd", "-o", exe]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"building {exe} failed:\n{r.stdout}")
