@@ -302,6 +302,13 @@ lmw_status lmw_corrector_set_scales(lmw_corrector *c, const float *in_scale, con
 lmw_status lmw_corrector_loss_grad(lmw_corrector *c, const float *x, const float *target, int nsamples, double *loss, float *grad);
 /* the correction for nsamples inputs, and the rms of the network's outputs per sample (residual may be NULL) */
 lmw_status lmw_corrector_forward(lmw_corrector *c, const float *x, int nsamples, float *correction, double *residual);
+/* lmw_corrector_forward on DEVICE arrays, allocating nothing: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  x and
+ * correction (and residual, which may be NULL) are device pointers of the sizes lmw_corrector_forward takes; the
+ * activations live in a static 32 MiB device workspace per GPU, held until the process ends, so nsamples is limited
+ * to what fits (T42, width 64: 40 samples); LMW_EINVAL says how many bytes a larger call would need.  Synchronous;
+ * calls that overlap on one GPU share the workspace and race. */
+lmw_status lmw_corrector_forward_noalloc(lmw_corrector *c, const float *x, int nsamples, float *correction, double *residual);
 /* the training set, resident on the device: inputs and target corrections */
 lmw_status lmw_corrector_set_dataset(lmw_corrector *c, const float *x, const float *target, int nsamples);
 /* `epochs` passes of Adam (lr, beta 0.9/0.999) over the training set in shuffled mini-batches; losses[e] = mean

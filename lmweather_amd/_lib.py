@@ -122,6 +122,7 @@ SIGNATURES = {
     "lmw_corrector_set_scales": [_vp, _fp, _fp, C.c_size_t],
     "lmw_corrector_loss_grad": [_vp, _fp, _fp, C.c_int, _dp, _fp],
     "lmw_corrector_forward": [_vp, _fp, C.c_int, _fp, _dp],
+    "lmw_corrector_forward_noalloc": [_vp, _vp, C.c_int, _vp, _vp],  # device pointers
     "lmw_corrector_set_dataset": [_vp, _fp, _fp, C.c_int],
     "lmw_corrector_train": [_vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _dp],
     "lmw_sphere_corrected_step": [_vp, _vp, C.c_double, C.c_int, C.c_int, C.c_double, _ip, _dp],

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "sphere_core.h"
@@ -233,8 +234,10 @@ __global__ __launch_bounds__(256) void k_sum_rows(const float *__restrict__ v, s
   if (threadIdx.x == 0) out[slot] = red[0];
 }
 
-// sum of squares of the trained outputs of each sample (network units): one block per sample
-__global__ __launch_bounds__(256) void k_member_sq(const float *__restrict__ Y, int C, const int *__restrict__ ordm, double *__restrict__ res) {
+// sum of squares of the trained outputs of each sample (network units): one block per sample; with per > 0 the rms
+// sqrt(sum / per) instead
+__global__ __launch_bounds__(256) void k_member_sq(const float *__restrict__ Y, int C, const int *__restrict__ ordm, double per,
+                                                   double *__restrict__ res) {
   __shared__ double red[256];
   const int s = blockIdx.x;
   double acc = 0;
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(256) void k_member_sq(const float *__restrict__ Y, 
     if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) res[s] = red[0];
+  if (threadIdx.x == 0) res[s] = per > 0 ? sqrt(red[0] / per) : red[0];
 }
 
 // correction in raw units: out[s][f][c][h] = Y[s][c][2f+h] * out_scale[f][n], zero on masked entries
@@ -486,20 +489,42 @@ struct Corrector {
     hipLaunchKernelGGL(k_dense_fwd, dim3(blocks(R, 64)), dim3(256), 0, stream, Xin, R, I, wb.p + oW, bias, O, mix, deg.p, C, N1, act, Yout);
     LMW_LAUNCH_CHECK();
   }
-  // P = mean_m(H), PM = P V^T (per sample and degree)
-  void mixing(const float *H, int S, size_t oV, float *P) {
+  // the activations of one forward pass of S samples: the handle's own buffers (ensure), or carved out of a
+  // caller-free workspace (forward_workspace)
+  struct Acts {
+    float *X, *H1, *H2, *H3, *Y, *P1, *P2, *PM;
+  };
+  Acts own() { return {X.p, H1.p, H2.p, H3.p, Y.p, P1.p, P2.p, PM.p}; }
+  // bytes of workspace the activations of a forward pass of S samples need, each array aligned to 256 bytes; with
+  // base != nullptr, the arrays' addresses inside [base, base + bytes)
+  size_t forward_workspace(int S, unsigned char *base, Acts *a) const {
+    const size_t R = static_cast<size_t>(S) * C, RW = R * W, PW = static_cast<size_t>(S) * N1 * W;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+      unsigned char *p = base ? base + o : nullptr;
+      o += (bytes + 255) / 256 * 256;
+      return p;
+    };
+    float **dst[8] = {&a->X, &a->H1, &a->H2, &a->H3, &a->Y, &a->P1, &a->P2, &a->PM};
+    const size_t count[8] = {R * FIN, RW, RW, RW, R * FOUT, PW, PW, PW};
+    for (int i = 0; i < 8; i++) *dst[i] = reinterpret_cast<float *>(take(count[i] * sizeof(float)));
+    return o;
+  }
+  // P = mean_m(H), PMo = P V^T (per sample and degree)
+  void mixing(const float *H, int S, size_t oV, float *P, float *PMo) {
     hipLaunchKernelGGL(k_pool, dim3(blocks(static_cast<size_t>(S) * N1 * W)), dim3(256), 0, stream, H, S, C, W, N1, P);
     LMW_LAUNCH_CHECK();
-    dense(P, S * N1, W, oV, nullptr, W, nullptr, 0, PM.p);
+    dense(P, S * N1, W, oV, nullptr, W, nullptr, 0, PMo);
   }
-  void forward(int S) {
+  void forward(int S) { forward(S, own()); }
+  void forward(int S, const Acts &a) {
     const int R = S * C;
-    dense(X.p, R, FIN, oW1, w.p + ob1, W, nullptr, 1, H1.p);
-    mixing(H1.p, S, oV2, P1.p);
-    dense(H1.p, R, W, oW2, w.p + ob2, W, PM.p, 1, H2.p);
-    mixing(H2.p, S, oV3, P2.p);
-    dense(H2.p, R, W, oW3, w.p + ob3, W, PM.p, 1, H3.p);
-    dense(H3.p, R, W, oW4, w.p + ob4, FOUT, nullptr, 0, Y.p);
+    dense(a.X, R, FIN, oW1, w.p + ob1, W, nullptr, 1, a.H1);
+    mixing(a.H1, S, oV2, a.P1, a.PM);
+    dense(a.H1, R, W, oW2, w.p + ob2, W, a.PM, 1, a.H2);
+    mixing(a.H2, S, oV3, a.P2, a.PM);
+    dense(a.H2, R, W, oW3, w.p + ob3, W, a.PM, 1, a.H3);
+    dense(a.H3, R, W, oW4, w.p + ob4, FOUT, nullptr, 0, a.Y);
   }
 
   // g[oW..] = dZ^T X and g[ob..] = column sums of dZ (ob == SIZE_MAX: no bias)
@@ -565,8 +590,9 @@ struct Corrector {
     hipLaunchKernelGGL(k_adam, dim3(blocks(np)), dim3(256), 0, stream, w.p, wb.p, g.p, m1.p, m2.p, np, lr, b1, b2, 1e-8f, bc1, bc2);
     LMW_LAUNCH_CHECK();
   }
-  void features(const float *rawp, int S) {
-    hipLaunchKernelGGL(k_features, dim3(blocks(static_cast<size_t>(S) * C)), dim3(256), 0, stream, rawp, S, C, N, deg.p, ordm.p, in_scale.p, FIN, X.p);
+  void features(const float *rawp, int S) { features(rawp, S, X.p); }
+  void features(const float *rawp, int S, float *Xo) {
+    hipLaunchKernelGGL(k_features, dim3(blocks(static_cast<size_t>(S) * C)), dim3(256), 0, stream, rawp, S, C, N, deg.p, ordm.p, in_scale.p, FIN, Xo);
     LMW_LAUNCH_CHECK();
   }
   void targets(const float *rawp, int S) {
@@ -576,20 +602,24 @@ struct Corrector {
   }
   // correction (raw units) of the S states in raw, into raw2; residual[s] = rms of the trained network outputs
   void correction(int S, double *residual) {
-    features(raw.p, S);
-    forward(S);
-    hipLaunchKernelGGL(k_unnormalize, dim3(blocks(static_cast<size_t>(S) * C)), dim3(256), 0, stream, Y.p, S, C, N, deg.p, ordm.p, out_scale.p,
-                       raw2.p);
-    LMW_LAUNCH_CHECK();
+    if (residual) need_sums(S);
+    correction(S, raw.p, raw2.p, own(), residual ? sums.p : nullptr);
     if (residual) {
-      need_sums(S);
-      hipLaunchKernelGGL(k_member_sq, dim3(S), dim3(256), 0, stream, Y.p, C, ordm.p, sums.p);
-      LMW_LAUNCH_CHECK();
-      std::vector<double> r(S);
-      LMW_HIP(hipMemcpyAsync(r.data(), sums.p, S * sizeof(double), hipMemcpyDeviceToHost, stream));
+      LMW_HIP(hipMemcpyAsync(residual, sums.p, S * sizeof(double), hipMemcpyDeviceToHost, stream));
       sync();
-      const double per = trained_count(1);
-      for (int s = 0; s < S; s++) residual[s] = std::sqrt(r[s] / per);
+    }
+  }
+  // the same on device arrays: states `in`, correction `out`, activations `a`; the per-sample rms into the device
+  // array `rms` when it is not null.  Launches only (on the handle's stream).
+  void correction(int S, const float *in, float *out, const Acts &a, double *rms) {
+    features(in, S, a.X);
+    forward(S, a);
+    hipLaunchKernelGGL(k_unnormalize, dim3(blocks(static_cast<size_t>(S) * C)), dim3(256), 0, stream, a.Y, S, C, N, deg.p, ordm.p, out_scale.p,
+                       out);
+    LMW_LAUNCH_CHECK();
+    if (rms) {
+      hipLaunchKernelGGL(k_member_sq, dim3(S), dim3(256), 0, stream, a.Y, C, ordm.p, trained_count(1), rms);
+      LMW_LAUNCH_CHECK();
     }
   }
 };
@@ -665,9 +695,48 @@ void truncated_in(SphereCore &c, int T, const float *in, size_t n) {
   c.sync();
 }
 
+// ---------------------------------------------------------------- the allocation-free forward pass
+// Workspace of lmw_corrector_forward_noalloc: the activations of one forward pass, in a static device array that
+// each GPU holds until the process ends (its own copy, at its own address).  Host code never uses the array's name
+// as an address: the per-device address comes from hipGetSymbolAddress, taken once per device with that device
+// current.  Every kernel of the pass writes each workspace element before reading it, so nothing is cleared between
+// calls; two calls that overlap on one GPU share the array and race.
+constexpr size_t kNoAllocBytes = static_cast<size_t>(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+unsigned char *scratch_address[kMaxDevices] = {};
+
+unsigned char *scratch_on_current_device() {
+  int dev = 0;
+  LMW_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error(LMW_EDEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[dev]) {
+    void *p = nullptr;
+    LMW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_address[dev] = static_cast<unsigned char *>(p);
+  }
+  return scratch_address[dev];
+}
+
+// true when p is device (or managed) memory; a query that fails leaves no error behind for later launch checks
+bool device_pointer(const void *p) {
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+}  // namespace
+
 }  // namespace lmw
 
 using namespace lmw;
+
 
 struct lmw_corrector {
   std::unique_ptr<Corrector> c;
@@ -847,6 +916,24 @@ lmw_status lmw_ensemble_set_truncated(lmw_ensemble *e, int truncation, const flo
   return guard([&] {
     NEED(e && in);
     truncated_in(single_group(e), truncation, in, n);
+  });
+}
+
+lmw_status lmw_corrector_forward_noalloc(lmw_corrector *c, const float *x, int nsamples, float *correction, double *residual) {
+  return guard([&] {
+    NEED(c && x && correction);
+    Corrector &k = *c->c;
+    if (nsamples < 1) invalid("need at least one sample");
+    if (!device_pointer(x) || !device_pointer(correction) || (residual && !device_pointer(residual)))
+      invalid("lmw_corrector_forward_noalloc: inputs and outputs must be device memory");
+    Corrector::Acts a{};
+    const size_t need = k.forward_workspace(nsamples, nullptr, &a);
+    if (need > kNoAllocBytes)
+      invalid("lmw_corrector_forward_noalloc: " + std::to_string(nsamples) + " samples need " + std::to_string(need) +
+              " bytes of workspace, more than the " + std::to_string(kNoAllocBytes) + " bytes it has");
+    k.forward_workspace(nsamples, scratch_on_current_device(), &a);
+    k.correction(nsamples, x, correction, a, residual);
+    k.sync();
   });
 }
 
