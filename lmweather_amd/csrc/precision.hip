@@ -79,7 +79,7 @@ __device__ inline void split8(const float *p, v8bf &a1, v8bf &a2, v8bf &a3) {
 
 // ------------------------------------------------------------------------------------------- table builders
 
-// Forward-transform A operand: FP32 [m][k = jh (P) | Jhp + jh (H)][r], split BF16 parts 1 and 2 [m][r][k] (BF16X3),
+// Forward-transform A operand: FP32 [m][k = 2 jh (P) | 2 jh + 1 (H)][r], split BF16 parts 1 and 2 [m][r][k] (BF16X3),
 // and FP32 [m][r][k] in that order (Abf: BF16X6 splits it into three parts as it loads), from the FP64 analysis
 // tables Pa/Ha [m][jh][r].
 __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa, const double *__restrict__ Ha, float *__restrict__ A32,
@@ -89,8 +89,8 @@ __global__ void k_build_anal_tables(Geo G, int R, const double *__restrict__ Pa,
     // e enumerates (global row gr, k) with k fastest: gr = poff[m] + r
     const int gr = static_cast<int>(e / (2 * G.Jhp)), k = static_cast<int>(e % (2 * G.Jhp));
     const int m = mrow[gr], r = gr - G.poff[m], kp = G.kpe[m] + G.kpo[m];
-    const int jh = k < G.Jhp ? k : k - G.Jhp;
-    const double *T = k < G.Jhp ? Pa : Ha;
+    const int jh = k >> 1;  // k = 2 jh + part: the P and H entries of a latitude pair side by side
+    const double *T = (k & 1) ? Ha : Pa;
     const float v = static_cast<float>(T[static_cast<size_t>(G.poff[m]) * G.Jhp + static_cast<size_t>(jh) * kp + r]);
     A32[static_cast<size_t>(2 * G.Jhp) * G.poff[m] + static_cast<size_t>(k) * kp + r] = v;
     __bf16 x1, x2, x3;
@@ -351,6 +351,42 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   constexpr int NIT = ((KC / 2) * MEM + 64 * NW - 1) / (64 * NW);
   ST vv[NIT][2][8];
   auto stage_load = [&](int c0) {
+    if constexpr (PREC != LMW_FP64) {
+      // mixed: K runs over (latitude pair, P | H) interleaved, so the thread of pair kk / 2 reads that pair's one
+      // 80-byte record (five weighted coefficients per latitude) and forms both its P operands X (k = kk) and its
+      // H operands Y (k = kk + 1): parity 0 (even n - m) takes X from north + south and Y from north - south,
+      // parity 1 the reverse (P(-mu) = (-1)^(n-m) P(mu), H(-mu) = -(-1)^(n-m) H(mu)).
+      const float sx = p ? -1.f : 1.f;
+#pragma unroll
+      for (int it = 0; it < NIT; it++) {
+        const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
+        const int jh = (c0 + kk) >> 1;
+        float4 r[5] = {};
+        if (e < (KC / 2) * MEM && bl < nb && jh < G.Jh) {
+          const float4 *src = reinterpret_cast<const float4 *>(Gb + ((static_cast<size_t>(b0 + bl) * G.M1 + m) * G.Jh + jh) * 10);
+#pragma unroll
+          for (int c = 0; c < 5; c++) r[c] = src[c];
+        }
+        // north: A r0.xy, B r0.zw, UP r1.xy, VP r1.zw, E r2.xy; south: A r2.zw, B r3.xy, UP r3.zw, VP r4.xy, E r4.zw
+        const float dm = static_cast<float>(m);
+        const float Ax = r[0].x + sx * r[2].z, Ay = r[0].y + sx * r[2].w;    // X combination (P rows)
+        const float Bx = r[0].z + sx * r[3].x, By = r[0].w + sx * r[3].y;
+        const float Ux = r[1].x + sx * r[3].z, Uy = r[1].y + sx * r[3].w;
+        const float Ex = r[2].x + sx * r[4].z, Ey = r[2].y + sx * r[4].w;
+        const float ax = r[0].x - sx * r[2].z, ay = r[0].y - sx * r[2].w;    // Y combination (H rows)
+        const float bx = r[0].z - sx * r[3].x, by = r[0].w - sx * r[3].y;
+        const float vx = r[1].z - sx * r[4].x, vy = r[1].w - sx * r[4].y;
+        // X: -i m A, i m B, E, -i m U phi; Y: B, A, 0, V phi (sphere.cpp:374-383); i m z = (-m z.y, m z.x)
+        const float X[8] = {dm * Ay, -(dm * Ax), -(dm * By), dm * Bx, Ex, Ey, dm * Uy, -(dm * Ux)};
+        const float Y[8] = {bx, by, ax, ay, 0.f, 0.f, vx, vy};
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          vv[it][0][q] = X[q];
+          vv[it][1][q] = Y[q];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
       const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
@@ -598,9 +634,9 @@ template <int PREC, int KC, int NW>
 __global__ __launch_bounds__(512) void k_anal_mx(Geo G, AnalTabs T, const AnalIn<PREC> *__restrict__ Gb, int mode, int stage, double dt, double nu4,
                                                  double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
                                                  double2 *__restrict__ raw, int b0, int nb, int mixed, int tiles) {
-  int mp, t;
-  if (!xcd_item(2 * G.M1, tiles, mp, t)) return;
-  anal_mx_block<PREC, KC, NW>(G, T, Gb, mode, stage, dt, nu4, x0, xs, acc, raw, b0, nb, mixed, mp >> 1, mp & 1, t * MEM);
+  int m, pt;
+  if (!xcd_item(G.M1, 2 * tiles, m, pt)) return;  // the two parities and the member tiles of one m on one XCD
+  anal_mx_block<PREC, KC, NW>(G, T, Gb, mode, stage, dt, nu4, x0, xs, acc, raw, b0, nb, mixed, m, pt & 1, (pt >> 1) * MEM);
 }
 
 // ------------------------------------------------------------------------------------------- multi-group kernels
@@ -733,7 +769,7 @@ template <int PREC, int KC, int NW>
 static void anal_mx_launch(SphereCore &c, int mode, int stage, double dt, int b0, int nb, double2 *raw_out) {
   AnalTabs T{c.A32.p, c.Abh.p, c.Abl.p, c.Abf.p, c.A64.p};
   const int tiles = (nb + MEM - 1) / MEM;
-  dim3 grd((2 * c.M1 + 7) / 8 * 8 * tiles), blk(64 * NW);
+  dim3 grd((c.M1 + 7) / 8 * 8 * 2 * tiles), blk(64 * NW);
   const size_t lds = PREC == LMW_FP64   ? 8 * KC * MEM * sizeof(double)
                      : PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float)
                                         : nsplit(PREC) * 8 * (KC + 8) * MEM * sizeof(__bf16);

@@ -435,6 +435,18 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
     C c[10];
 #pragma unroll
     for (int r = 0; r < 10; r++) c[r] = rfft_coef<F>(buf + r * P, m, tw, invK);
+    if constexpr (sizeof(TS) == 8) {
+      // FP32: one 80-byte record per (member, m, latitude pair) of the five Fourier coefficients at both latitudes,
+      // weighted (wd A, wd B, wd U phi, wd V phi, wj E; north then south); the forward transform forms the i m
+      // factors and the north/south combinations of each row parity as it loads them (five stored values per
+      // latitude instead of eight operands per parity)
+      float4 *o = reinterpret_cast<float4 *>(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 10);
+      const C v[10] = {cscale(c[0], wd), cscale(c[2], wd), cscale(c[4], wd), cscale(c[6], wd), cscale(c[8], wj),
+                       cscale(c[1], wd), cscale(c[3], wd), cscale(c[5], wd), cscale(c[7], wd), cscale(c[9], wj)};
+#pragma unroll
+      for (int i = 0; i < 5; i++) o[i] = make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+      continue;
+    }
     const T dm = static_cast<T>(m);
     // dzeta -= (i m A wd) P - (B wd) H; ddelta += (i m B wd) P + (A wd) H + (E w) P n(n+1)/a^2;
     // dphi -= (i m U phi wd) P - (V phi wd) H   (sphere.cpp:374-383); index 0 north, 1 south

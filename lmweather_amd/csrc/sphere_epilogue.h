@@ -43,10 +43,18 @@ __device__ inline void tend_store(const Geo &G, int stage, double dt, double nu4
       fv = exp(-nu4 * kv * kv * dt);
       fpp = exp(-nu4 * kq * kq * dt);
     }
+    // The (0, 0) coefficients have zero tendency and no filter, so every stage leaves their state unchanged: only
+    // their RK4 sum is written.  Nothing in the launch rewrites xs at q = 0, so every block may read phi_00 of the
+    // stage input (the mean height of the mixed path's phibar) without a race.
+    const bool keep = q == 0;
 #pragma unroll
     for (int fld = 0; fld < 3; fld++) {
       const size_t i = G.at(b, fld, q);
       const double2 s0 = t.s0[fld], kv = kk[fld];
+      if (keep) {
+        if (stage <= 3) acc[i] = stage == 1 ? kv : cadd(t.a[fld], cscale(kv, 2.0));
+        continue;
+      }
       if (stage == 1) {
         acc[i] = kv;
         xs[i] = cadd(s0, cscale(kv, 0.5 * dt));
