@@ -28,6 +28,9 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 namespace {
 constexpr int MEM = 16;  // members per block (the N dimension of an MFMA tile)
+// the inverse transform's northern latitudes are split over this many blocks per (m, member tile): more, smaller
+// blocks, so that one block's coefficient staging overlaps another's MFMAs
+constexpr int kSynthJSplit = 2;
 
 __device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
   hi = static_cast<__bf16>(x);
@@ -167,7 +170,7 @@ struct SynthTabs {
 // The work of one block (order m, members mb .. mb + 15); shared by the per-group and the multi-group kernels.
 template <int PREC, int RC>
 __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T, const double2 *__restrict__ spec, float2 *__restrict__ Fs,
-                                               int b0, int nb, int mixed, int m, int mb) {
+                                               int b0, int nb, int mixed, int m, int mb, int jt0 = 0) {
   constexpr int SP = RC + 8;  // padded row pitch (bf16) of the BF16 chunk: distinct LDS banks across members
   extern __shared__ float lds_f[];
   float *cf = lds_f;                                                       // FP32 chunk [q][k'][member]
@@ -176,7 +179,7 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int Km = G.N - m + 1, kpe = G.kpe[m], kpo = G.kpo[m];
   const double a = G.a;
-  const int jt = wave;  // row tile (16 northern latitudes) of this wave
+  const int jt = jt0 + wave;  // row tile (16 northern latitudes) of this wave
   const bool active = jt * 16 < G.Jh;
   v4f accP[2][8], accH[2][4];
   for (int p = 0; p < 2; p++) {
@@ -626,8 +629,9 @@ template <int PREC, int RC>
 __global__ __launch_bounds__(512) void k_synth_mx(Geo G, SynthTabs T, const double2 *__restrict__ spec, float2 *__restrict__ Fs, int b0,
                                                   int nb, int mixed, int tiles) {
   int m, t;
-  if (!xcd_item4(G.M1, tiles, m, t)) return;
-  synth_mx_block<PREC, RC>(G, T, spec, Fs, b0, nb, mixed, m, t * MEM);
+  if (!xcd_item4(G.M1, tiles * kSynthJSplit, m, t)) return;
+  // member tile t / kSynthJSplit; the block's waves take the row tiles of latitude part t % kSynthJSplit
+  synth_mx_block<PREC, RC>(G, T, spec, Fs, b0, nb, mixed, m, (t / kSynthJSplit) * MEM, (t % kSynthJSplit) * (blockDim.x >> 6));
 }
 
 template <int PREC, int KC, int NW>
@@ -743,9 +747,9 @@ const int kAnalNW[kAnalTiles] = {4, 4, 4, 8, 8, 8};
 template <int PREC, int RC>
 static void synth_mx_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
   SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Spf.p, c.Shf.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
-  const int waves = (c.Jh + 15) / 16;
+  const int waves = ((c.Jh + 15) / 16 + kSynthJSplit - 1) / kSynthJSplit;  // row tiles of one latitude part
   const int tiles = (nb + MEM - 1) / MEM;
-  dim3 grd(((c.M1 + 3) / 4 + 7) / 8 * 8 * 4 * tiles), blk(64 * waves);
+  dim3 grd(((c.M1 + 3) / 4 + 7) / 8 * 8 * 4 * tiles * kSynthJSplit), blk(64 * waves);
   const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : nsplit(PREC) * 8 * (RC + 8) * MEM * sizeof(__bf16);
   hipLaunchKernelGGL((k_synth_mx<PREC, RC>), grd, blk, lds, c.stream, c.geo(), T, spec, c.Fs32.p, b0, nb, 1, tiles);
   LMW_LAUNCH_CHECK();
