@@ -299,6 +299,47 @@ def accuracy_bench(lmw, args) -> dict:
             "wall_s": time.perf_counter() - t0, "accuracy": res}
 
 
+def diagnostics_bench(lmw, args) -> dict:
+    """Diagnostics and the checks of run() on one T170 member (the C2 state): each scalar diagnostic through the
+    C ABI (one device synthesis of the grids, one compensated reduction, the scalar copied to the host), and run()
+    against steps() over the same 640 steps (run checks finiteness and the stability limit every 64 steps)."""
+    h, u, v, hs = initial_state(lmw)
+    s = lmw.Sphere(N, lmw.SphereParams(A, OM, G))
+    s.set_orography_grid(hs)
+    s.set_state_grid(h, u, v)
+    s.energy()
+    out = {}
+    for name in ("energy", "potential_enstrophy", "max_wind", "height_error_l2", "mean_height"):
+        fn = getattr(s, name)
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(50):
+            fn()
+        out[f"{name}_ms"] = 1e3 * (time.perf_counter() - t0) / 50
+    t0 = time.perf_counter()
+    for _ in range(20):
+        s.stable_dt()
+    out["stable_dt_ms"] = 1e3 * (time.perf_counter() - t0) / 20
+    n = 640
+    s.steps(DT, 64)
+    walls = {}
+    for mode in ("steps", "run", "steps", "run"):
+        s.set_state_grid(h, u, v)
+        t0 = time.perf_counter()
+        if mode == "steps":
+            s.steps(DT, n)
+        else:
+            s.run(n * DT, DT)
+        w = time.perf_counter() - t0
+        walls[mode] = min(walls.get(mode, w), w)
+    return {"metric": "diagnostics and run() checks, single T170 instance", "value": out["energy_ms"], "unit": "ms per energy()",
+            "n_gpus": 1, "steps": n, "warmup": 64, "ms_per_step": 1e3 * walls["steps"] / n, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "T170 single instance (512x256 Gaussian grid), C2 state over 8 Gaussian mountains"},
+            "diagnostics": out, "steps_s": walls["steps"], "run_s": walls["run"],
+            "run_overhead_frac": walls["run"] / walls["steps"] - 1.0}
+
+
 def plane_jet_state(nx, ny, lx, ly, g, H, f0, beta, U=1.0, Lw=1.0e5):
     """Geostrophically balanced sech^2 jet: u = U sech^2(s) at u points, h = H - (f0 U L / g) tanh s
     - (beta U L^2 / g)(s tanh s - ln cosh s) at centres, s = (y - ly/2) / L (config C4)."""
@@ -530,7 +571,7 @@ def setup_bench(lmw, args):
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup", "c1", "accuracy"])
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup", "c1", "accuracy", "diagnostics"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=4)
@@ -563,6 +604,9 @@ def main() -> int:
         return 0
     if args.workload == "accuracy":
         print(json.dumps(accuracy_bench(lmw, args)))
+        return 0
+    if args.workload == "diagnostics":
+        print(json.dumps(diagnostics_bench(lmw, args)))
         return 0
     prec = PREC_NAMES.index(args.precision)
     mixed = prec != lmw.FP64

@@ -91,6 +91,7 @@ struct SphereCore {
   DevArray<double> in3;                        // [b][3][J][K] input grids for the analysis path
   DevArray<double2> spec_io;                   // [b][4][S] raw transform outputs / inputs
   DevArray<double> red;                        // [b][16] reductions
+  DevArray<double> rowred;                     // [b][9][J] per-row partial reductions
   DevArray<int> mflags;                        // [b] bit 0 untilted axis, bit 1 no orography, bits 2.. hs plane
   DevArray<float2> Fs32, Gb32;                 // FP32 storage of Fs and Gb for the mixed-precision step
   // passive tracer q (single-member spheres only): state, RK4 stage state and accumulator [S], the spectral input

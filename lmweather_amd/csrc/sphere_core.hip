@@ -653,15 +653,18 @@ struct Kahan {
 };
 }  // namespace
 
-// Area means and extremes of one member (sphere.cpp:438-447,514-590), rows in parallel, rows combined in order.
-// red[b][0..8] = kinetic, potential, potential enstrophy, max wind, max h, err num, err den, max |h - ref|, max |ref|.
-__global__ __launch_bounds__(256) void k_reduce(Geo G, const double *__restrict__ grid, const double *__restrict__ fgrid,
-                                                const double *__restrict__ hs, const double *__restrict__ ref_phi,
-                                                const double *__restrict__ w, double *__restrict__ red, int b0, const int *__restrict__ mflags) {
-  const int b = b0 + blockIdx.x, j = threadIdx.x, K = G.K;
-  __shared__ double rows[9][256];
-  const size_t plane = static_cast<size_t>(G.J) * K;
+// Area means and extremes (sphere.cpp:438-447,514-590) in two launches.  k_reduce_rows: one wavefront per latitude
+// row (four rows per block), lanes along the longitudes with coalesced loads, each lane's Neumaier sums combined over
+// the wavefront in lane order; rowred[b][q][j] for the 9 quantities q = kinetic, potential, potential enstrophy,
+// max wind, max h, err num, err den, max |h - ref|, max |ref|.  k_reduce_combine: the rows of one member in order,
+// as the reference sums them, into red[b][0..8].
+__global__ __launch_bounds__(256) void k_reduce_rows(Geo G, const double *__restrict__ grid, const double *__restrict__ fgrid,
+                                                     const double *__restrict__ hs, const double *__restrict__ ref_phi,
+                                                     double *__restrict__ rowred, int b0, const int *__restrict__ mflags) {
+  const int b = b0 + blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = 4 * blockIdx.x + wave, K = G.K;
+  __shared__ double part[4][9][65];
   if (j < G.J) {
+    const size_t plane = static_cast<size_t>(G.J) * K;
     const double *hh = grid + static_cast<size_t>(b) * 4 * plane + static_cast<size_t>(j) * K;
     const double *uu = hh + plane, *vv = hh + 2 * plane, *zz = hh + 3 * plane;
     const double *ff = fgrid + static_cast<size_t>(b) * plane + static_cast<size_t>(j) * K;
@@ -670,7 +673,7 @@ __global__ __launch_bounds__(256) void k_reduce(Geo G, const double *__restrict_
     Kahan ke, pe, en, num, den;
     double wmax = 0, hmax = 0, mn = 0, mr = 0;
     const double g = G.g;
-    for (int k = 0; k < K; k++) {
+    for (int k = lane; k < K; k += 64) {
       const double h = hh[k], u = uu[k], v = vv[k], z = zz[k] + ff[k];
       ke.add(0.5 * h * (u * u + v * v));
       pe.add(0.5 * g * h * h + g * h * ss[k]);
@@ -683,31 +686,41 @@ __global__ __launch_bounds__(256) void k_reduce(Geo G, const double *__restrict_
       mn = fmax(mn, fabs(d));
       mr = fmax(mr, fabs(rf));
     }
-    rows[0][j] = ke.value();
-    rows[1][j] = pe.value();
-    rows[2][j] = en.value();
-    rows[3][j] = wmax;
-    rows[4][j] = hmax;
-    rows[5][j] = num.value();
-    rows[6][j] = den.value();
-    rows[7][j] = mn;
-    rows[8][j] = mr;
+    const double v9[9] = {ke.value(), pe.value(), en.value(), wmax, hmax, num.value(), den.value(), mn, mr};
+#pragma unroll
+    for (int q = 0; q < 9; q++) part[wave][q][lane] = v9[q];
   }
   __syncthreads();
-  if (j < 9) {
-    // one thread per quantity, over all rows in order: 0..2 area means (x 1/2, sphere.cpp:520-533), 3, 4 maxima,
-    // 5, 6 weighted sums (sphere.cpp:579-580), 7, 8 maxima
-    double out;
-    if (j < 3 || j == 5 || j == 6) {
-      Kahan s;
-      for (int r = 0; r < G.J; r++) s.add(j < 3 ? w[r] * rows[j][r] / K : w[r] * rows[j][r]);
-      out = j < 3 ? 0.5 * s.value() : s.value();
+  if (j < G.J && lane < 9) {
+    const int q = lane;
+    double out = 0;
+    if (q == 3 || q == 4 || q == 7 || q == 8) {
+      for (int l = 0; l < 64; l++) out = fmax(out, part[wave][q][l]);
     } else {
-      out = 0;
-      for (int r = 0; r < G.J; r++) out = fmax(out, rows[j][r]);
+      Kahan s;
+      for (int l = 0; l < 64; l++) s.add(part[wave][q][l]);
+      out = s.value();
     }
-    red[static_cast<size_t>(b) * 16 + j] = out;
+    rowred[(static_cast<size_t>(b) * 9 + q) * G.J + j] = out;
   }
+}
+
+__global__ __launch_bounds__(64) void k_reduce_combine(Geo G, const double *__restrict__ rowred, const double *__restrict__ w,
+                                                       double *__restrict__ red, int b0) {
+  const int b = b0 + blockIdx.x, q = threadIdx.x, K = G.K;
+  if (q >= 9) return;
+  // 0..2 area means (x 1/2, sphere.cpp:520-533), 3, 4 maxima, 5, 6 weighted sums (sphere.cpp:579-580), 7, 8 maxima
+  const double *rows = rowred + (static_cast<size_t>(b) * 9 + q) * G.J;
+  double out;
+  if (q < 3 || q == 5 || q == 6) {
+    Kahan s;
+    for (int r = 0; r < G.J; r++) s.add(q < 3 ? w[r] * rows[r] / K : w[r] * rows[r]);
+    out = q < 3 ? 0.5 * s.value() : s.value();
+  } else {
+    out = 0;
+    for (int r = 0; r < G.J; r++) out = fmax(out, rows[r]);
+  }
+  red[static_cast<size_t>(b) * 16 + q] = out;
 }
 
 // Sum of |coefficient| over zeta, delta, phi of one member (the finiteness check of sphere.cpp:449-455).
@@ -902,6 +915,7 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
   in3.alloc(3 * plane);
   spec_io.alloc(4 * static_cast<size_t>(S));
   red.alloc(static_cast<size_t>(B) * 16);
+  rowred.alloc(static_cast<size_t>(B) * 9 * J);
   t.assign(B, 0.0);
   tilt.assign(B, 0.0);
   has_orography.assign(B, 0);
@@ -1453,7 +1467,9 @@ void SphereCore::compute_grids(int b0, int nb) {
 }
 
 void SphereCore::reduce(int b0, int nb) {
-  hipLaunchKernelGGL(k_reduce, dim3(nb), dim3(256), 0, stream, geo(), grid.p, f.p, hs.p, ref_phi.p, w.p, red.p, b0, mflags.p);
+  hipLaunchKernelGGL(k_reduce_rows, dim3((J + 3) / 4, nb), dim3(256), 0, stream, geo(), grid.p, f.p, hs.p, ref_phi.p, rowred.p, b0, mflags.p);
+  LMW_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_combine, dim3(nb), dim3(64), 0, stream, geo(), rowred.p, w.p, red.p, b0);
   LMW_LAUNCH_CHECK();
 }
 
@@ -1535,7 +1551,50 @@ __global__ __launch_bounds__(256) void k_fft_batch(double2 *data, int n, int log
   for (int i = threadIdx.x; i < n; i += blockDim.x) row[i] = buf[i];
 }
 
+// The same transforms through the grid kernel's four-step FFT (fft_lds.h), for the lengths the time step uses (the
+// complex half-length L = K / 2 of its real rows, 8..256): natural-order input in LDS, fft2, output read back from
+// its logical positions.  tw[q] = exp(-2 pi i q / (2 n)), q < 2 n, as the grid kernel's table.
+template <int R1, int R2>
+__global__ __launch_bounds__(256) void k_fft2_batch(double2 *data, const double2 *__restrict__ twg, int sign) {
+  using F = Fft2<R1, R2>;
+  constexpr int n = F::L;
+  extern __shared__ double2 lds[];
+  double2 *buf = lds, *tw = lds + F::P;
+  double2 *row = data + static_cast<size_t>(blockIdx.x) * n;
+  for (int i = threadIdx.x; i < 2 * n; i += blockDim.x) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) buf[F::phys(i)] = row[i];
+  __syncthreads();
+  if (sign > 0) fft2<R1, R2, +1>(buf, 1, tw);
+  else fft2<R1, R2, -1>(buf, 1, tw);
+  for (int t = threadIdx.x; t < n; t += blockDim.x) row[t] = buf[F::phys(F::pos(t))];
+}
+
 void fft_batch(double2 *host, int n, int count, int sign) {
+  if (n >= 8 && n <= 256) {  // the production four-step FFT
+    std::vector<double2> htw(2 * n);
+    for (int q = 0; q < 2 * n; q++) {
+      const long double ang = 3.141592653589793238462643383279502884L * q / n;
+      htw[q] = make_double2(static_cast<double>(cosl(ang)), static_cast<double>(-sinl(ang)));
+    }
+    DevArray<double2> d, t;
+    d.alloc(static_cast<size_t>(n) * count);
+    t.alloc(2 * n);
+    LMW_HIP(hipMemcpy(d.p, host, d.bytes(), hipMemcpyHostToDevice));
+    LMW_HIP(hipMemcpy(t.p, htw.data(), t.bytes(), hipMemcpyHostToDevice));
+    const size_t pitch = static_cast<size_t>(n) + n / (n >= 64 ? 8 : n >= 16 ? 4 : 2);  // Fft2::P of the dispatched shape
+    const size_t lds = (pitch + 2 * static_cast<size_t>(n)) * sizeof(double2);
+    switch (n) {
+      case 256: hipLaunchKernelGGL((k_fft2_batch<16, 16>), dim3(count), dim3(256), lds, 0, d.p, t.p, sign); break;
+      case 128: hipLaunchKernelGGL((k_fft2_batch<16, 8>), dim3(count), dim3(256), lds, 0, d.p, t.p, sign); break;
+      case 64: hipLaunchKernelGGL((k_fft2_batch<8, 8>), dim3(count), dim3(256), lds, 0, d.p, t.p, sign); break;
+      case 32: hipLaunchKernelGGL((k_fft2_batch<8, 4>), dim3(count), dim3(256), lds, 0, d.p, t.p, sign); break;
+      case 16: hipLaunchKernelGGL((k_fft2_batch<4, 4>), dim3(count), dim3(256), lds, 0, d.p, t.p, sign); break;
+      default: hipLaunchKernelGGL((k_fft2_batch<4, 2>), dim3(count), dim3(256), lds, 0, d.p, t.p, sign); break;
+    }
+    LMW_LAUNCH_CHECK();
+    LMW_HIP(hipMemcpy(host, d.p, d.bytes(), hipMemcpyDeviceToHost));
+    return;
+  }
   int logn = 0;
   while ((1 << logn) < n) logn++;
   std::vector<double2> htw(n / 2);

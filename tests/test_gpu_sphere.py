@@ -43,11 +43,15 @@ def test_gauss_legendre(lmw, oracle):
 
 
 def test_fft_matches_oracle_and_dft(lmw, oracle):
+    """lmw_fft runs the grid kernel's four-step FFT (fft_lds.h: fft2) for the lengths 8..256 that the time step uses
+    and the radix-2 kernel beyond them; both against the oracle's fft (sphere.cpp:21-45) and a direct DFT."""
     rng = np.random.default_rng(1)
-    for n in (16, 128, 256, 512):
+    for n in (8, 16, 32, 64, 128, 256, 512):
         a = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
         for sign in (-1, 1):
             got = lmw.fft(a, sign)
+            dft = np.fft.fft(a, axis=1) if sign < 0 else n * np.fft.ifft(a, axis=1)  # unnormalized, e^(sign 2 pi i nk/N)
+            assert np.max(np.abs(got - dft)) <= 1e-13 * n
             for r in range(3):
                 ref = oracle.fft(a[r], sign)
                 assert np.max(np.abs(got[r] - ref)) <= 1e-13 * n
