@@ -300,7 +300,10 @@ class Plane {
     return dvdx - (uj - ujm) / dy();
   }
   // the Plane.fields operator (cpp/main.cpp:410-431): h = H + h(x, y), u, v from expressions
-  void fields(const std::string &h, const std::string &u, const std::string &v) { check(lmw_plane_fields(h_, 0, h.c_str(), u.c_str(), v.c_str())); }
+  void fields(const std::string &h, const std::string &u, const std::string &v) {
+    check(lmw_plane_fields(h_, 0, h.c_str(), u.c_str(), v.c_str()));
+    pull();
+  }
   // the text files of the Plane.save_csv / save_row / save_column operators (cpp/main.cpp:464-493)
   void save_csv(const std::string &path) const { check(lmw_plane_save_csv(h_, 0, path.c_str())); }
   void save_row(int row, const std::string &path) const { check(lmw_plane_save_row(h_, 0, row, path.c_str())); }

@@ -62,18 +62,25 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
-def build_cpp_tests() -> str:
-    """Compile tests/facade_checks.cpp (closed-form checks of the C++ facade include/lmw.hpp) into
-    tests/bin/facade_checks, linked against liblmw_hip.so."""
+CPP_TESTS = ("facade_checks", "ops_checks")
+
+
+def build_cpp_tests() -> list[str]:
+    """Compile the native test programs into tests/bin, linked against liblmw_hip.so: tests/facade_checks.cpp
+    (closed-form checks of the C++ facade include/lmw.hpp) and tests/ops_checks.cpp (the operator table
+    include/lmw_ops.hpp: schema, signatures, every operator once)."""
     out_dir = os.path.join(ROOT, "tests", "bin")
     os.makedirs(out_dir, exist_ok=True)
-    exe = os.path.join(out_dir, "facade_checks")
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
-           os.path.join(ROOT, "tests", "facade_checks.cpp"), "-L", HERE, "-llmw_hip", "-Wl,-rpath,$ORIGIN/../../lmweather_amd", "-o", exe]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"building {exe} failed:\n{r.stdout}")
-    return exe
+    built = []
+    for name in CPP_TESTS:
+        exe = os.path.join(out_dir, name)
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+               os.path.join(ROOT, "tests", name + ".cpp"), "-L", HERE, "-llmw_hip", "-Wl,-rpath,$ORIGIN/../../lmweather_amd", "-o", exe]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {exe} failed:\n{r.stdout}")
+        built.append(exe)
+    return built
 
 
 def _drain(procs) -> None:
