@@ -59,6 +59,12 @@ struct SphereCore {
   lmw_sphere_params prm;
   int precision = LMW_FP64;
   hipStream_t stream = nullptr;
+  // member parts of a large group stepped as concurrent branches of the step (parts of whole 16-member tiles): the
+  // second and later parts run on these streams, forked from and joined to `stream` with these events
+  static constexpr int kMaxParts = 4;
+  hipStream_t part_stream[kMaxParts] = {};
+  hipEvent_t part_fork = nullptr, part_join[kMaxParts] = {};
+  int step_parts() const;
   std::vector<int> h_moff, h_poff, h_kpe, h_kpo;
   // per member host state
   std::vector<double> t;                      // model time
@@ -150,7 +156,7 @@ struct SphereCore {
   void drop_graph();
 
   // stepping (all members)
-  void launch_stage(int stage, double dt);
+  void launch_stage(int stage, double dt, int b0, int nb);
   void enqueue_step(double dt);
   void steps(double dt, int64_t count);
   void steps_graph(double dt, int64_t count);

@@ -937,6 +937,11 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
 
 SphereCore::~SphereCore() {
   if (graph) (void)hipGraphExecDestroy(graph);
+  for (int p = 1; p < kMaxParts; p++) {
+    if (part_stream[p]) (void)hipStreamDestroy(part_stream[p]);
+    if (part_join[p]) (void)hipEventDestroy(part_join[p]);
+  }
+  if (part_fork) (void)hipEventDestroy(part_fork);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1284,23 +1289,58 @@ void SphereCore::mark_reference_range(int b0, int nb) {
   if (has_tracer) tracer_grid_device(*this, ref_q.p);  // sphere.cpp:238
 }
 
-void SphereCore::launch_stage(int stage, double dt) {
+void SphereCore::launch_stage(int stage, double dt, int b0, int nb) {
   if (precision == LMW_FP64) {
-    launch_synth(xs.p, 0, B);
-    launch_tend_grid(0, B, 0);
-    launch_anal(ANAL_TEND, stage, dt, 0, B, nullptr);
+    launch_synth(xs.p, b0, nb);
+    launch_tend_grid(b0, nb, 0);
+    launch_anal(ANAL_TEND, stage, dt, b0, nb, nullptr);
   } else {  // only the Legendre contractions of the time step change precision
-    launch_synth_mx(xs.p, 0, B);
-    launch_tend_grid(0, B, 1);
-    launch_anal_mx(ANAL_TEND, stage, dt, 0, B, nullptr);
+    launch_synth_mx(xs.p, b0, nb);
+    launch_tend_grid(b0, nb, 1);
+    launch_anal_mx(ANAL_TEND, stage, dt, b0, nb, nullptr);
   }
 }
 
+// Members are independent, so a group of several 16-member tiles is stepped as up to kMaxParts parts of whole tiles,
+// each part's four stages a branch of its own: one part's kernels fill the gaps (tails, synchronisation phases) of
+// another's.  Single members and tracers keep one branch.
+int SphereCore::step_parts() const {
+  const int tiles = (B + 15) / 16;
+  return has_tracer || advect_only ? 1 : std::min(tiles, kMaxParts);
+}
+
 void SphereCore::enqueue_step(double dt) {
+  const int parts = step_parts();
+  if (parts > 1) {
+    const int tiles = (B + 15) / 16;
+    if (!part_fork) {
+      LMW_HIP(hipEventCreateWithFlags(&part_fork, hipEventDisableTiming));
+      for (int p = 1; p < kMaxParts; p++) {
+        LMW_HIP(hipStreamCreate(&part_stream[p]));
+        LMW_HIP(hipEventCreateWithFlags(&part_join[p], hipEventDisableTiming));
+      }
+    }
+    LMW_HIP(hipEventRecord(part_fork, stream));
+    const hipStream_t main = stream;
+    for (int p = parts - 1; p >= 0; p--) {
+      const int b0 = 16 * (p * tiles / parts), b1 = std::min(B, 16 * ((p + 1) * tiles / parts));
+      if (p > 0) {
+        LMW_HIP(hipStreamWaitEvent(part_stream[p], part_fork, 0));
+        stream = part_stream[p];  // the launch functions enqueue on `stream`
+      }
+      for (int stage = 1; stage <= 4; stage++) launch_stage(stage, dt, b0, b1 - b0);
+      if (p > 0) {
+        LMW_HIP(hipEventRecord(part_join[p], stream));
+        stream = main;
+      }
+    }
+    for (int p = 1; p < parts; p++) LMW_HIP(hipStreamWaitEvent(stream, part_join[p], 0));
+    return;
+  }
   // the tracer tendency reads the stage state before the dynamics' stage kernel advances it (sphere.cpp:412-427)
   for (int stage = 1; stage <= 4; stage++) {
     if (has_tracer) launch_tracer_tendency();
-    if (!advect_only) launch_stage(stage, dt);
+    if (!advect_only) launch_stage(stage, dt, 0, B);
     if (has_tracer) launch_tracer_update(stage, dt);
   }
   if (advect_only && prm.diffusion > 0) launch_state_filter(dt);
