@@ -573,8 +573,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup", "c1", "accuracy", "diagnostics"])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--members", type=int, default=32)
     ap.add_argument("--precision", default="bf16x6", choices=list(PREC_NAMES))
     ap.add_argument("--check-days", type=float, default=1.0)
