@@ -340,6 +340,39 @@ def diagnostics_bench(lmw, args) -> dict:
             "run_overhead_frac": walls["run"] / walls["steps"] - 1.0}
 
 
+def corrector_bench(lmw, args) -> dict:
+    """The learned corrector's kernels (T42, width 64): training throughput (shuffled mini-batches of 32 samples from a
+    device-resident set of 2048 synthetic samples, forward + backward + Adam per mini-batch) and inference throughput
+    (the forward pass of 256 samples from host arrays), in rows (sample x coefficient) per second.  value = training
+    rows/s."""
+    from lmweather_amd.corrector import Corrector
+    rng = np.random.default_rng(5)
+    net = Corrector(42, 64, seed=1)
+    x = rng.standard_normal((2048, 3, net.C, 2)).astype(np.float32)
+    t = (0.1 * rng.standard_normal((2048, 3, net.C, 2))).astype(np.float32)
+    net.set_dataset(x, t)
+    net.train(1, 32, 1e-3, seed=1)  # warm-up: buffers, first launches
+    epochs = 3
+    t0 = time.perf_counter()
+    losses = net.train(epochs, 32, 1e-3, seed=2)
+    train_s = time.perf_counter() - t0
+    rows_train = epochs * 2048 * net.C / train_s
+    xi = x[:256]
+    net.forward(xi)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        net.forward(xi)
+    fwd_s = (time.perf_counter() - t0) / 5
+    return {"metric": "learned corrector training rows/s (T42, width 64)", "value": rows_train, "unit": "rows/s",
+            "n_gpus": 1, "steps": epochs * 2048 // 32, "warmup": 2048 // 32, "ms_per_step": 1e3 * train_s / (epochs * 2048 // 32),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 weights, activations split into three bf16 parts on MFMA (f32-exact products), f32 accumulation",
+            "data": "synthetic", "config": {"workload": "corrector T42 width 64: 2048 synthetic samples, batch 32, 3 epochs; "
+                                                        "inference 256 samples"},
+            "inference_rows_per_s": 256 * net.C / fwd_s, "inference_ms_256_samples": 1e3 * fwd_s,
+            "final_epoch_loss": float(losses[-1])}
+
+
 def plane_jet_state(nx, ny, lx, ly, g, H, f0, beta, U=1.0, Lw=1.0e5):
     """Geostrophically balanced sech^2 jet: u = U sech^2(s) at u points, h = H - (f0 U L / g) tanh s
     - (beta U L^2 / g)(s tanh s - ln cosh s) at centres, s = (y - ly/2) / L (config C4)."""
@@ -571,7 +604,7 @@ def setup_bench(lmw, args):
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup", "c1", "accuracy", "diagnostics"])
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "plane", "ensemble", "setup", "c1", "accuracy", "diagnostics", "corrector"])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
@@ -607,6 +640,9 @@ def main() -> int:
         return 0
     if args.workload == "diagnostics":
         print(json.dumps(diagnostics_bench(lmw, args)))
+        return 0
+    if args.workload == "corrector":
+        print(json.dumps(corrector_bench(lmw, args)))
         return 0
     prec = PREC_NAMES.index(args.precision)
     mixed = prec != lmw.FP64

@@ -72,6 +72,128 @@ __global__ __launch_bounds__(256) void k_dense_fwd(const float *__restrict__ X, 
   }
 }
 
+// ---------------------------------------------------------------- dense layers on the BF16 MFMA (16x16x32)
+// The weights used for the arithmetic are BF16 values already (wb); each FP32 activation or gradient x is split into
+// three BF16 parts x1 + x2 + x3 = x exactly, so the three products w x1 + w x2 + w x3 are exact and the FP32
+// accumulation gives the FP32 sum of the exact products (in the MFMA's order).  Lane l supplies A[l % 16][8 (l / 16)
+// .. + 7] and B[8 (l / 16) .. + 7][l % 16]; it receives D[4 (l / 16) + i][l % 16] in register i.
+typedef float mx_v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 mx_v8bf __attribute__((ext_vector_type(8)));
+
+__device__ inline void split3(const float (&x)[8], mx_v8bf &a1, mx_v8bf &a2, mx_v8bf &a3) {
+#pragma unroll
+  for (int t = 0; t < 8; t++) {
+    const __bf16 h = static_cast<__bf16>(x[t]);
+    const float r1 = x[t] - static_cast<float>(h);
+    const __bf16 m = static_cast<__bf16>(r1);
+    a1[t] = h;
+    a2[t] = m;
+    a3[t] = static_cast<__bf16>(r1 - static_cast<float>(m));
+  }
+}
+__device__ inline mx_v4f mma3x(const mx_v8bf &a1, const mx_v8bf &a2, const mx_v8bf &a3, const mx_v8bf &w, mx_v4f c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, w, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, w, c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, w, c, 0, 0, 0);
+}
+
+// Y[r][o] = act(sum_i X[r][i] W[o][i] + b[o] + mix[(r / C) * N1 + deg[r % C]][o]) for I a multiple of 32 and
+// O <= 16 NCT.  Block: 4 waves, one 16-row tile each; a wave computes all NCT column tiles of its rows.
+template <int NCT>
+__global__ __launch_bounds__(256) void k_dense_fwd_mx(const float *__restrict__ X, int R, int I, const float *__restrict__ Wb,
+                                                      const float *__restrict__ b, int O, const float *__restrict__ mix,
+                                                      const int *__restrict__ deg, int C, int N1, int act, float *__restrict__ Y) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r0 = (blockIdx.x * 4 + wave) * 16;
+  if (r0 >= R) return;
+  const int c16 = lane & 15, kq = lane >> 4;
+  mx_v4f acc[NCT];
+#pragma unroll
+  for (int t = 0; t < NCT; t++) acc[t] = mx_v4f{0, 0, 0, 0};
+  const int ra = r0 + c16;
+  for (int k0 = 0; k0 < I; k0 += 32) {
+    float x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (ra < R) {
+      const float4 *p = reinterpret_cast<const float4 *>(X + static_cast<size_t>(ra) * I + k0 + 8 * kq);
+      const float4 u = p[0], v = p[1];
+      x[0] = u.x, x[1] = u.y, x[2] = u.z, x[3] = u.w, x[4] = v.x, x[5] = v.y, x[6] = v.z, x[7] = v.w;
+    }
+    mx_v8bf x1, x2, x3;
+    split3(x, x1, x2, x3);
+#pragma unroll
+    for (int t = 0; t < NCT; t++) {
+      const int o = 16 * t + c16;
+      mx_v8bf w;
+      if (o < O) {
+        const float4 *q = reinterpret_cast<const float4 *>(Wb + static_cast<size_t>(o) * I + k0 + 8 * kq);
+        const float4 u = q[0], v = q[1];
+        w = mx_v8bf{static_cast<__bf16>(u.x), static_cast<__bf16>(u.y), static_cast<__bf16>(u.z), static_cast<__bf16>(u.w),
+                    static_cast<__bf16>(v.x), static_cast<__bf16>(v.y), static_cast<__bf16>(v.z), static_cast<__bf16>(v.w)};
+      } else {
+        w = mx_v8bf{0, 0, 0, 0, 0, 0, 0, 0};
+      }
+      acc[t] = mma3x(x1, x2, x3, w, acc[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NCT; t++) {
+    const int o = 16 * t + c16;
+    if (o >= O) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = r0 + 4 * kq + i;
+      if (r >= R) continue;
+      float v = acc[t][i] + (b ? b[o] : 0.f);
+      if (mix) v += mix[(static_cast<size_t>(r / C) * N1 + deg[r % C]) * O + o];
+      Y[static_cast<size_t>(r) * O + o] = act ? tanhf(v) : v;
+    }
+  }
+}
+
+// dX[r][i] = sum_o dZ[r][o] W[o][i] for I a multiple of 16 (I / 16 <= NCT column tiles) and O <= 64: W transposed
+// into LDS as BF16 [i][o], o padded to a multiple of 32 with zeros.
+template <int NCT>
+__global__ __launch_bounds__(256) void k_dense_bwd_data_mx(const float *__restrict__ dZ, int R, int O, const float *__restrict__ Wb, int I,
+                                                           float *__restrict__ dX) {
+  __shared__ __bf16 wt[16 * NCT][WMAX + 8];
+  const int OP = (O + 31) / 32 * 32;
+  for (int e = threadIdx.x; e < 16 * NCT * OP; e += blockDim.x) {
+    const int i = e / OP, o = e % OP;
+    wt[i][o] = static_cast<__bf16>(i < I && o < O ? Wb[static_cast<size_t>(o) * I + i] : 0.f);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r0 = (blockIdx.x * 4 + wave) * 16;
+  if (r0 >= R) return;
+  const int c16 = lane & 15, kq = lane >> 4, ra = r0 + c16;
+  mx_v4f acc[NCT];
+#pragma unroll
+  for (int t = 0; t < NCT; t++) acc[t] = mx_v4f{0, 0, 0, 0};
+  for (int k0 = 0; k0 < OP; k0 += 32) {
+    float x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int o = k0 + 8 * kq + u;
+      x[u] = ra < R && o < O ? dZ[static_cast<size_t>(ra) * O + o] : 0.f;
+    }
+    mx_v8bf x1, x2, x3;
+    split3(x, x1, x2, x3);
+#pragma unroll
+    for (int t = 0; t < NCT; t++) {
+      const mx_v8bf w = *reinterpret_cast<const mx_v8bf *>(&wt[16 * t + c16][k0 + 8 * kq]);
+      acc[t] = mma3x(x1, x2, x3, w, acc[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NCT; t++) {
+    const int i = 16 * t + c16;
+    if (i >= I) continue;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int r = r0 + 4 * kq + q;
+      if (r < R) dX[static_cast<size_t>(r) * I + i] = acc[t][q];
+    }
+  }
+}
+
 // dX[r][i] = sum_o dZ[r][o] W[o][i]
 __global__ __launch_bounds__(256) void k_dense_bwd_data(const float *__restrict__ dZ, int R, int O, const float *__restrict__ Wb, int I,
                                                         float *__restrict__ dX) {
@@ -485,8 +607,20 @@ struct Corrector {
   }
 
   // ---------------------------------------------------------------- network
+  // dense layers with inputs in multiples of 32 run on the BF16 MFMA, others (the 8 input features) on FMA
   void dense(const float *Xin, int R, int I, size_t oW, const float *bias, int O, const float *mix, int act, float *Yout) {
-    hipLaunchKernelGGL(k_dense_fwd, dim3(blocks(R, 64)), dim3(256), 0, stream, Xin, R, I, wb.p + oW, bias, O, mix, deg.p, C, N1, act, Yout);
+    const float *Wl = wb.p + oW;
+    if (I % 32 == 0) {
+      const dim3 g(blocks(R, 64));
+      switch ((O + 15) / 16) {
+        case 1: hipLaunchKernelGGL(k_dense_fwd_mx<1>, g, dim3(256), 0, stream, Xin, R, I, Wl, bias, O, mix, deg.p, C, N1, act, Yout); break;
+        case 2: hipLaunchKernelGGL(k_dense_fwd_mx<2>, g, dim3(256), 0, stream, Xin, R, I, Wl, bias, O, mix, deg.p, C, N1, act, Yout); break;
+        case 3: hipLaunchKernelGGL(k_dense_fwd_mx<3>, g, dim3(256), 0, stream, Xin, R, I, Wl, bias, O, mix, deg.p, C, N1, act, Yout); break;
+        default: hipLaunchKernelGGL(k_dense_fwd_mx<4>, g, dim3(256), 0, stream, Xin, R, I, Wl, bias, O, mix, deg.p, C, N1, act, Yout); break;
+      }
+    } else {
+      hipLaunchKernelGGL(k_dense_fwd, dim3(blocks(R, 64)), dim3(256), 0, stream, Xin, R, I, Wl, bias, O, mix, deg.p, C, N1, act, Yout);
+    }
     LMW_LAUNCH_CHECK();
   }
   // the activations of one forward pass of S samples: the handle's own buffers (ensure), or carved out of a
@@ -537,7 +671,18 @@ struct Corrector {
     LMW_LAUNCH_CHECK();
   }
   void bwd_data(const float *dZ, int R, int O, size_t oW, int I, float *dX) {
-    hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks(R, 64)), dim3(256), 0, stream, dZ, R, O, wb.p + oW, I, dX);
+    const float *Wl = wb.p + oW;
+    if (I % 16 == 0) {
+      const dim3 g(blocks(R, 64));
+      switch (I / 16) {
+        case 1: hipLaunchKernelGGL(k_dense_bwd_data_mx<1>, g, dim3(256), 0, stream, dZ, R, O, Wl, I, dX); break;
+        case 2: hipLaunchKernelGGL(k_dense_bwd_data_mx<2>, g, dim3(256), 0, stream, dZ, R, O, Wl, I, dX); break;
+        case 3: hipLaunchKernelGGL(k_dense_bwd_data_mx<3>, g, dim3(256), 0, stream, dZ, R, O, Wl, I, dX); break;
+        default: hipLaunchKernelGGL(k_dense_bwd_data_mx<4>, g, dim3(256), 0, stream, dZ, R, O, Wl, I, dX); break;
+      }
+    } else {
+      hipLaunchKernelGGL(k_dense_bwd_data, dim3(blocks(R, 64)), dim3(256), 0, stream, dZ, R, O, Wl, I, dX);
+    }
     LMW_LAUNCH_CHECK();
   }
   // backward of an inner layer: dZ (holding dH on entry) -> weight, bias and mixing gradients, and
