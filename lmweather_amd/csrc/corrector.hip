@@ -194,6 +194,78 @@ __global__ __launch_bounds__(256) void k_dense_bwd_data_mx(const float *__restri
   }
 }
 
+// The weight-gradient partial sums of k_dense_bwd_weight on the BF16 MFMA: part[blk][o I + i] = sum over the WROWS
+// rows of block blk of dZ[r][o] X[r][i], part[blk][O I + o] = sum dZ[r][o].  Both operands are FP32 and split into
+// three BF16 parts; the six products x1 y1 + x1 y2 + x2 y1 + x1 y3 + x2 y2 + x3 y1 keep FP32 accuracy (the dropped
+// terms are below 2^-24 relative).  Rows are staged 32 at a time in LDS, transposed to [o][r] and [i][r], so that a
+// lane's 8 consecutive k (rows) are contiguous.  Block: 4 waves over the (O / 16) x (I / 16) output tiles.
+__global__ __launch_bounds__(256) void k_dense_bwd_weight_mx(const float *__restrict__ dZ, const float *__restrict__ X, int R, int O, int I,
+                                                             float *__restrict__ part) {
+  constexpr int RC = 32, PT = RC + 8;  // rows per chunk, padded pitch (bf16)
+  __shared__ __bf16 zt[3][WMAX][PT], xt[3][WMAX][PT];
+  __shared__ float bsum[4][WMAX];
+  const int blk = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rb = blk * WROWS, re = min(R, rb + WROWS);
+  const int OT = (O + 15) / 16, IT = (I + 15) / 16, ntile = OT * IT;
+  mx_v4f acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = mx_v4f{0, 0, 0, 0};
+  float bs = 0;  // thread t < O: the bias sum of output t over this wave's rows of the chunk (4 waves, combined below)
+  for (int c0 = rb; c0 < re; c0 += RC) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < RC * WMAX; e += blockDim.x) {
+      const int rr = e / WMAX, c = e % WMAX, r = c0 + rr;
+      const float z = r < re && c < O ? dZ[static_cast<size_t>(r) * O + c] : 0.f;
+      const float x = r < re && c < I ? X[static_cast<size_t>(r) * I + c] : 0.f;
+      const __bf16 z1 = static_cast<__bf16>(z), x1 = static_cast<__bf16>(x);
+      const float zr = z - static_cast<float>(z1), xr = x - static_cast<float>(x1);
+      const __bf16 z2 = static_cast<__bf16>(zr), x2 = static_cast<__bf16>(xr);
+      zt[0][c][rr] = z1;
+      zt[1][c][rr] = z2;
+      zt[2][c][rr] = static_cast<__bf16>(zr - static_cast<float>(z2));
+      xt[0][c][rr] = x1;
+      xt[1][c][rr] = x2;
+      xt[2][c][rr] = static_cast<__bf16>(xr - static_cast<float>(x2));
+    }
+    __syncthreads();
+    if (threadIdx.x < O)
+      for (int rr = 0; rr < RC && c0 + rr < re; rr++) bs += dZ[static_cast<size_t>(c0 + rr) * O + threadIdx.x];
+    const int kk = 8 * (lane >> 4), c16 = lane & 15;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int t = wave + 4 * q;
+      if (t >= ntile) break;
+      const int o = 16 * (t / IT) + c16, i = 16 * (t % IT) + c16;
+      const mx_v8bf a1 = *reinterpret_cast<const mx_v8bf *>(&zt[0][o][kk]), a2 = *reinterpret_cast<const mx_v8bf *>(&zt[1][o][kk]),
+                    a3 = *reinterpret_cast<const mx_v8bf *>(&zt[2][o][kk]);
+      const mx_v8bf b1 = *reinterpret_cast<const mx_v8bf *>(&xt[0][i][kk]), b2 = *reinterpret_cast<const mx_v8bf *>(&xt[1][i][kk]),
+                    b3 = *reinterpret_cast<const mx_v8bf *>(&xt[2][i][kk]);
+      mx_v4f c = acc[q];
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b1, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b2, c, 0, 0, 0);
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
+    }
+  }
+  const int nw = O * I + O;
+  float *out = part + static_cast<size_t>(blk) * nw;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int t = wave + 4 * q;
+    if (t >= ntile) break;
+    const int i = 16 * (t % IT) + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int o = 16 * (t / IT) + 4 * (lane >> 4) + r;
+      if (o < O && i < I) out[static_cast<size_t>(o) * I + i] = acc[q][r];
+    }
+  }
+  if (threadIdx.x < O) out[O * I + threadIdx.x] = bs;
+  (void)bsum;
+}
+
 // dX[r][i] = sum_o dZ[r][o] W[o][i]
 __global__ __launch_bounds__(256) void k_dense_bwd_data(const float *__restrict__ dZ, int R, int O, const float *__restrict__ Wb, int I,
                                                         float *__restrict__ dX) {
@@ -664,7 +736,10 @@ struct Corrector {
   // g[oW..] = dZ^T X and g[ob..] = column sums of dZ (ob == SIZE_MAX: no bias)
   void wgrad(const float *dZ, const float *Xin, int R, int O, int I, size_t oW, size_t ob) {
     const int nblk = (R + WROWS - 1) / WROWS;
-    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nblk), dim3(256), 0, stream, dZ, Xin, R, O, I, part.p);
+    if (O % 16 == 0 && I % 16 == 0 && (O / 16) * (I / 16) <= 16)
+      hipLaunchKernelGGL(k_dense_bwd_weight_mx, dim3(nblk), dim3(256), 0, stream, dZ, Xin, R, O, I, part.p);
+    else
+      hipLaunchKernelGGL(k_dense_bwd_weight, dim3(nblk), dim3(256), 0, stream, dZ, Xin, R, O, I, part.p);
     LMW_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_sum_parts, dim3(blocks(static_cast<size_t>(O) * I + O)), dim3(256), 0, stream, part.p, nblk, O, I, g.p + oW,
                        ob == SIZE_MAX ? nullptr : g.p + ob);
