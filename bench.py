@@ -78,14 +78,17 @@ def build_batch(lmw, members, precision, state):
 def kernel_costs(J, K, S, M1, members, mixed, orography_planes=1):
     """Algorithmic work per launch (one RK4 stage of the batch).  Legendre flops as executed with the equatorial
     folding (half of the reference's 24 J S and 28 J S per member); for split BF16 the MFMA executes three products
-    per multiply-add.  Grid kernel HBM bytes: Fourier coefficients in and analysis operands out per member (FP32
-    storage in the mixed modes, FP64 otherwise), and each distinct orography grid once (members with the same
-    orography share one device plane; the benchmark's members all share one)."""
+    per multiply-add.  Grid kernel HBM bytes: the four Fourier coefficients of every (latitude, m) in, the analysis
+    operands out -- in the mixed modes one FP32 record of ten complex values (five weighted coefficients at each
+    latitude of the pair) per (m, latitude pair), in FP64 sixteen complex operands (both parities' north/south
+    combinations) -- per member, and each distinct orography grid once (members with the same orography share one
+    device plane; the benchmark's members all share one)."""
     Jh = J // 2
     cb = 8 if mixed else 16  # bytes per stored complex
+    per_pair = 10 if mixed else 16  # stored complex values per (m, latitude pair)
     synth_flops = 24.0 * Jh * S * members
     anal_flops = 28.0 * Jh * S * members
-    grid_bytes = (4 * J * M1 * cb + M1 * Jh * 16 * cb) * members + J * K * 8 * orography_planes
+    grid_bytes = (4 * J * M1 * cb + M1 * Jh * per_pair * cb) * members + J * K * 8 * orography_planes
     return synth_flops, grid_bytes, anal_flops
 
 
