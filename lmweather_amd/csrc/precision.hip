@@ -32,12 +32,16 @@ constexpr int MEM = 16;  // members per block (the N dimension of an MFMA tile)
 // blocks, so that one block's coefficient staging overlaps another's MFMAs
 constexpr int kSynthJSplit = 2;
 
+// The remainders are formed from the rounded FP32 value x, never fused with the product that produced x (FMA
+// contraction would split the exact product instead, and the bits would then depend on where the split is inlined).
 __device__ inline void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+#pragma clang fp contract(off)
   hi = static_cast<__bf16>(x);
   lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 // x = x1 + x2 + x3 to the last bit of an FP32 value (each remainder is exact in FP32)
 __device__ inline void split_bf16x3(float x, __bf16 &x1, __bf16 &x2, __bf16 &x3) {
+#pragma clang fp contract(off)
   x1 = static_cast<__bf16>(x);
   const float r1 = x - static_cast<float>(x1);
   x2 = static_cast<__bf16>(r1);
@@ -353,39 +357,21 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   // registers while the current chunk's MFMAs run.
   constexpr int NIT = ((KC / 2) * MEM + 64 * NW - 1) / (64 * NW);
   ST vv[NIT][2][8];
+  float4 rr[NIT][5];  // mixed: the raw records of the next chunk
   auto stage_load = [&](int c0) {
     if constexpr (PREC != LMW_FP64) {
-      // mixed: K runs over (latitude pair, P | H) interleaved, so the thread of pair kk / 2 reads that pair's one
-      // 80-byte record (five weighted coefficients per latitude) and forms both its P operands X (k = kk) and its
-      // H operands Y (k = kk + 1): parity 0 (even n - m) takes X from north + south and Y from north - south,
-      // parity 1 the reverse (P(-mu) = (-1)^(n-m) P(mu), H(-mu) = -(-1)^(n-m) H(mu)).
-      const float sx = p ? -1.f : 1.f;
+      // mixed: the thread of latitude pair kk / 2 loads that pair's one 80-byte record (five weighted coefficients
+      // per latitude) and keeps it raw until the chunk is written to LDS, so that nothing waits for the load here
 #pragma unroll
       for (int it = 0; it < NIT; it++) {
         const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
         const int jh = (c0 + kk) >> 1;
-        float4 r[5] = {};
+#pragma unroll
+        for (int c = 0; c < 5; c++) rr[it][c] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (e < (KC / 2) * MEM && bl < nb && jh < G.Jh) {
           const float4 *src = reinterpret_cast<const float4 *>(Gb + ((static_cast<size_t>(b0 + bl) * G.M1 + m) * G.Jh + jh) * 10);
 #pragma unroll
-          for (int c = 0; c < 5; c++) r[c] = src[c];
-        }
-        // north: A r0.xy, B r0.zw, UP r1.xy, VP r1.zw, E r2.xy; south: A r2.zw, B r3.xy, UP r3.zw, VP r4.xy, E r4.zw
-        const float dm = static_cast<float>(m);
-        const float Ax = r[0].x + sx * r[2].z, Ay = r[0].y + sx * r[2].w;    // X combination (P rows)
-        const float Bx = r[0].z + sx * r[3].x, By = r[0].w + sx * r[3].y;
-        const float Ux = r[1].x + sx * r[3].z, Uy = r[1].y + sx * r[3].w;
-        const float Ex = r[2].x + sx * r[4].z, Ey = r[2].y + sx * r[4].w;
-        const float ax = r[0].x - sx * r[2].z, ay = r[0].y - sx * r[2].w;    // Y combination (H rows)
-        const float bx = r[0].z - sx * r[3].x, by = r[0].w - sx * r[3].y;
-        const float vx = r[1].z - sx * r[4].x, vy = r[1].w - sx * r[4].y;
-        // X: -i m A, i m B, E, -i m U phi; Y: B, A, 0, V phi (sphere.cpp:374-383); i m z = (-m z.y, m z.x)
-        const float X[8] = {dm * Ay, -(dm * Ax), -(dm * By), dm * Bx, Ex, Ey, dm * Uy, -(dm * Ux)};
-        const float Y[8] = {bx, by, ax, ay, 0.f, 0.f, vx, vy};
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          vv[it][0][q] = X[q];
-          vv[it][1][q] = Y[q];
+          for (int c = 0; c < 5; c++) rr[it][c] = src[c];
         }
       }
       return;
@@ -408,13 +394,52 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       }
     }
   };
+  // BF16X6 with 32-wide chunks: this chunk's table operands are requested as the chunk starts, ahead of the next
+  // chunk's records (loads retire in order, so the MFMAs then wait for the tables only)
+  constexpr bool PIPE = PREC == LMW_BF16X6 && KC == 32;
+  v4f araw[TPW][2];
   stage_load(0);
   for (int c0 = 0; c0 < KK; c0 += KC) {
     __syncthreads();
+    if constexpr (PIPE) {
+#pragma unroll
+      for (int ti = 0; ti < TPW; ti++) {
+        const int t = wave + NW * ti;
+        if (t < nrt) {
+          const float *ap = T.Abf + static_cast<size_t>(KK) * (G.poff[m] + rbase + 16 * t + (lane & 15)) + c0 + 8 * (lane >> 4);
+          araw[ti][0] = *reinterpret_cast<const v4f *>(ap);
+          araw[ti][1] = *reinterpret_cast<const v4f *>(ap + 4);
+        }
+      }
+    }
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
       const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM);
       if (e >= (KC / 2) * MEM) continue;
+      if constexpr (PREC != LMW_FP64) {
+        // mixed: K runs over (latitude pair, P | H) interleaved, so the record of pair kk / 2 gives both its P
+        // operands X (k = kk) and its H operands Y (k = kk + 1): parity 0 (even n - m) takes X from north + south
+        // and Y from north - south, parity 1 the reverse (P(-mu) = (-1)^(n-m) P(mu), H(-mu) = -(-1)^(n-m) H(mu))
+        const float sx = p ? -1.f : 1.f;
+        const float4 *r = rr[it];
+        const float dm = static_cast<float>(m);
+        // north: A r0.xy, B r0.zw, UP r1.xy, VP r1.zw, E r2.xy; south: A r2.zw, B r3.xy, UP r3.zw, VP r4.xy, E r4.zw
+        const float Ax = r[0].x + sx * r[2].z, Ay = r[0].y + sx * r[2].w;  // X combination (P rows)
+        const float Bx = r[0].z + sx * r[3].x, By = r[0].w + sx * r[3].y;
+        const float Ux = r[1].x + sx * r[3].z, Uy = r[1].y + sx * r[3].w;
+        const float Ex = r[2].x + sx * r[4].z, Ey = r[2].y + sx * r[4].w;
+        const float ax = r[0].x - sx * r[2].z, ay = r[0].y - sx * r[2].w;  // Y combination (H rows)
+        const float bx = r[0].z - sx * r[3].x, by = r[0].w - sx * r[3].y;
+        const float vx = r[1].z - sx * r[4].x, vy = r[1].w - sx * r[4].y;
+        // X: -i m A, i m B, E, -i m U phi; Y: B, A, 0, V phi (sphere.cpp:374-383); i m z = (-m z.y, m z.x)
+        const float X[8] = {dm * Ay, -(dm * Ax), -(dm * By), dm * Bx, Ex, Ey, dm * Uy, -(dm * Ux)};
+        const float Y[8] = {bx, by, ax, ay, 0.f, 0.f, vx, vy};
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          vv[it][0][q] = X[q];
+          vv[it][1][q] = Y[q];
+        }
+      }
 #pragma unroll
       for (int q = 0; q < 8; q++) {
         if constexpr (PREC == LMW_FP32 || PREC == LMW_FP64) {
@@ -458,7 +483,18 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
           const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + r) + c0 + 32 * s + 8 * (lane >> 4);
           if constexpr (PREC == LMW_BF16X6) {
             v8bf ah, al, a3;
-            split8(T.Abf + ai, ah, al, a3);
+            if (PIPE) {
+#pragma unroll
+              for (int i = 0; i < 8; i++) {
+                __bf16 x1, x2, x3;
+                split_bf16x3(i < 4 ? araw[ti][0][i] : araw[ti][1][i - 4], x1, x2, x3);
+                ah[i] = x1;
+                al[i] = x2;
+                a3[i] = x3;
+              }
+            } else {
+              split8(T.Abf + ai, ah, al, a3);
+            }
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
