@@ -185,6 +185,7 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
   const double a = G.a;
   const int jt = jt0 + wave;  // row tile (16 northern latitudes) of this wave
   const bool active = jt * 16 < G.Jh;
+  constexpr bool PIPE = (PREC == LMW_BF16X6 || PREC == LMW_BF16X3) && RC == 32;
   v4f accP[2][8], accH[2][4];
   for (int p = 0; p < 2; p++) {
     for (int q = 0; q < 8; q++) accP[p][q] = v4f{0, 0, 0, 0};
@@ -195,6 +196,27 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
     const int kp32 = PREC != LMW_FP32 ? (p ? T.ko32[m] : T.ke32[m]) : kp;
     for (int c0 = 0; c0 < kp32; c0 += RC) {
       __syncthreads();
+      // split BF16 with 32-row chunks: this chunk's P and H table operands (FP32 entries for BF16X6, their two BF16
+      // parts for BF16X3) are requested before the coefficients are staged, so that they have arrived when the MFMAs
+      // start
+      v4f araw[4];
+      if constexpr (PIPE) {
+        if (active) {
+          const size_t base = static_cast<size_t>(T.soff[m]) + (p ? static_cast<size_t>(G.Jhp) * T.ke32[m] : 0) +
+                              static_cast<size_t>(jt * 16 + (lane & 15)) * kp32 + c0 + 8 * (lane >> 4);
+          if constexpr (PREC == LMW_BF16X6) {
+            araw[0] = *reinterpret_cast<const v4f *>(T.Spf + base);
+            araw[1] = *reinterpret_cast<const v4f *>(T.Spf + base + 4);
+            araw[2] = *reinterpret_cast<const v4f *>(T.Shf + base);
+            araw[3] = *reinterpret_cast<const v4f *>(T.Shf + base + 4);
+          } else {
+            araw[0] = *reinterpret_cast<const v4f *>(T.Sph + base);
+            araw[1] = *reinterpret_cast<const v4f *>(T.Spl + base);
+            araw[2] = *reinterpret_cast<const v4f *>(T.Shh + base);
+            araw[3] = *reinterpret_cast<const v4f *>(T.Shl + base);
+          }
+        }
+      }
       // stage 8 quantities x RC rows x 16 members: one thread per (member, pair of rows), members fastest, so
       // that a wavefront writes consecutive LDS words (FP32) or whole words of a padded pitch (BF16)
       for (int e = threadIdx.x; e < (RC / 2) * MEM; e += blockDim.x) {
@@ -261,8 +283,23 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
                               static_cast<size_t>(jh) * kp32 + c0 + 32 * s + 8 * (lane >> 4);
           if constexpr (PREC == LMW_BF16X6) {
             v8bf aPh, aPl, aP3, aHh, aHl, aH3;
-            split8(T.Spf + base, aPh, aPl, aP3);
-            split8(T.Shf + base, aHh, aHl, aH3);
+            if constexpr (PIPE) {
+#pragma unroll
+              for (int i = 0; i < 8; i++) {
+                __bf16 x1, x2, x3;
+                split_bf16x3(i < 4 ? araw[0][i] : araw[1][i - 4], x1, x2, x3);
+                aPh[i] = x1;
+                aPl[i] = x2;
+                aP3[i] = x3;
+                split_bf16x3(i < 4 ? araw[2][i] : araw[3][i - 4], x1, x2, x3);
+                aHh[i] = x1;
+                aHl[i] = x2;
+                aH3[i] = x3;
+              }
+            } else {
+              split8(T.Spf + base, aPh, aPl, aP3);
+              split8(T.Shf + base, aHh, aHl, aH3);
+            }
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
@@ -271,7 +308,18 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
               if (q >= 4) accH[p][q - 4] = mma6(aHh, aHl, aH3, bh, bl, b3, accH[p][q - 4]);
             }
           } else {
-            const v8bf aPh = load8(T.Sph + base), aPl = load8(T.Spl + base), aHh = load8(T.Shh + base), aHl = load8(T.Shl + base);
+            v8bf aPh, aPl, aHh, aHl;
+            if constexpr (PIPE) {
+              aPh = __builtin_bit_cast(v8bf, araw[0]);
+              aPl = __builtin_bit_cast(v8bf, araw[1]);
+              aHh = __builtin_bit_cast(v8bf, araw[2]);
+              aHl = __builtin_bit_cast(v8bf, araw[3]);
+            } else {
+              aPh = load8(T.Sph + base);
+              aPl = load8(T.Spl + base);
+              aHh = load8(T.Shh + base);
+              aHl = load8(T.Shl + base);
+            }
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * SP + 32 * s + 8 * (lane >> 4);
@@ -394,9 +442,10 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       }
     }
   };
-  // BF16X6 with 32-wide chunks: this chunk's table operands are requested as the chunk starts, ahead of the next
-  // chunk's records (loads retire in order, so the MFMAs then wait for the tables only)
-  constexpr bool PIPE = PREC == LMW_BF16X6 && KC == 32;
+  // split BF16 with 32-wide chunks: this chunk's table operands (FP32 entries for BF16X6, their two BF16 parts for
+  // BF16X3) are requested as the chunk starts, ahead of the next chunk's records (loads retire in order, so the MFMAs
+  // then wait for the tables only)
+  constexpr bool PIPE = (PREC == LMW_BF16X6 || PREC == LMW_BF16X3) && KC == 32;
   v4f araw[TPW][2];
   stage_load(0);
   for (int c0 = 0; c0 < KK; c0 += KC) {
@@ -406,9 +455,14 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       for (int ti = 0; ti < TPW; ti++) {
         const int t = wave + NW * ti;
         if (t < nrt) {
-          const float *ap = T.Abf + static_cast<size_t>(KK) * (G.poff[m] + rbase + 16 * t + (lane & 15)) + c0 + 8 * (lane >> 4);
-          araw[ti][0] = *reinterpret_cast<const v4f *>(ap);
-          araw[ti][1] = *reinterpret_cast<const v4f *>(ap + 4);
+          const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + rbase + 16 * t + (lane & 15)) + c0 + 8 * (lane >> 4);
+          if constexpr (PREC == LMW_BF16X6) {
+            araw[ti][0] = *reinterpret_cast<const v4f *>(T.Abf + ai);
+            araw[ti][1] = *reinterpret_cast<const v4f *>(T.Abf + ai + 4);
+          } else {
+            araw[ti][0] = *reinterpret_cast<const v4f *>(T.Abh + ai);
+            araw[ti][1] = *reinterpret_cast<const v4f *>(T.Abl + ai);
+          }
         }
       }
     }
@@ -483,7 +537,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
           const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + r) + c0 + 32 * s + 8 * (lane >> 4);
           if constexpr (PREC == LMW_BF16X6) {
             v8bf ah, al, a3;
-            if (PIPE) {
+            if constexpr (PIPE) {
 #pragma unroll
               for (int i = 0; i < 8; i++) {
                 __bf16 x1, x2, x3;
@@ -501,7 +555,14 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
               accv[ti][q] = mma6(ah, al, a3, load8(bh + bo), load8(blo + bo), load8(b3p + bo), accv[ti][q]);
             }
           } else {
-            const v8bf ah = load8(T.Abh + ai), al = load8(T.Abl + ai);
+            v8bf ah, al;
+            if constexpr (PIPE) {
+              ah = __builtin_bit_cast(v8bf, araw[ti][0]);
+              al = __builtin_bit_cast(v8bf, araw[ti][1]);
+            } else {
+              ah = load8(T.Abh + ai);
+              al = load8(T.Abl + ai);
+            }
 #pragma unroll
             for (int q = 0; q < 8; q++) {
               const int bo = (q * MEM + (lane & 15)) * AP + 32 * s + 8 * (lane >> 4);
