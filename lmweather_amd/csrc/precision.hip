@@ -406,21 +406,22 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   constexpr int NIT = ((KC / 2) * MEM + 64 * NW - 1) / (64 * NW);
   ST vv[NIT][2][8];
   float4 rr[NIT][5];  // mixed: the raw records of the next chunk
+  bool rr_ok[NIT];
   auto stage_load = [&](int c0) {
     if constexpr (PREC != LMW_FP64) {
       // mixed: the thread of latitude pair kk / 2 loads that pair's one 80-byte record (five weighted coefficients
       // per latitude) and keeps it raw until the chunk is written to LDS, so that nothing waits for the load here
+      // The loads are issued unconditionally (from a clamped, valid record; rr_ok marks the lanes whose record is
+      // used), so that every chunk issues the same loads in the same order and the waits for them stay exact.
 #pragma unroll
       for (int it = 0; it < NIT; it++) {
         const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
         const int jh = (c0 + kk) >> 1;
+        rr_ok[it] = e < (KC / 2) * MEM && bl < nb && jh < G.Jh;
+        const int blc = min(bl, nb - 1), jhc = min(jh, G.Jh - 1);
+        const float4 *src = reinterpret_cast<const float4 *>(Gb + ((static_cast<size_t>(b0 + blc) * G.M1 + m) * G.Jh + jhc) * 10);
 #pragma unroll
-        for (int c = 0; c < 5; c++) rr[it][c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (e < (KC / 2) * MEM && bl < nb && jh < G.Jh) {
-          const float4 *src = reinterpret_cast<const float4 *>(Gb + ((static_cast<size_t>(b0 + bl) * G.M1 + m) * G.Jh + jh) * 10);
-#pragma unroll
-          for (int c = 0; c < 5; c++) rr[it][c] = src[c];
-        }
+        for (int c = 0; c < 5; c++) rr[it][c] = src[c];
       }
       return;
     }
@@ -453,8 +454,8 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
     if constexpr (PIPE) {
 #pragma unroll
       for (int ti = 0; ti < TPW; ti++) {
-        const int t = wave + NW * ti;
-        if (t < nrt) {
+        const int t = min(wave + NW * ti, nrt - 1);  // (a wave without this tile loads a valid row it does not use)
+        {
           const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + rbase + 16 * t + (lane & 15)) + c0 + 8 * (lane >> 4);
           if constexpr (PREC == LMW_BF16X6) {
             araw[ti][0] = *reinterpret_cast<const v4f *>(T.Abf + ai);
@@ -475,7 +476,9 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
         // operands X (k = kk) and its H operands Y (k = kk + 1): parity 0 (even n - m) takes X from north + south
         // and Y from north - south, parity 1 the reverse (P(-mu) = (-1)^(n-m) P(mu), H(-mu) = -(-1)^(n-m) H(mu))
         const float sx = p ? -1.f : 1.f;
-        const float4 *r = rr[it];
+        float4 r[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) r[c] = rr_ok[it] ? rr[it][c] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float dm = static_cast<float>(m);
         // north: A r0.xy, B r0.zw, UP r1.xy, VP r1.zw, E r2.xy; south: A r2.zw, B r3.xy, UP r3.zw, VP r4.xy, E r4.zw
         const float Ax = r[0].x + sx * r[2].z, Ay = r[0].y + sx * r[2].w;  // X combination (P rows)
@@ -510,7 +513,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
         }
       }
     }
-    if (c0 + KC < KK) stage_load(c0 + KC);
+    stage_load(c0 + KC < KK ? c0 + KC : c0);  // (the last chunk requests itself again: the same load count)
     __syncthreads();
 #pragma unroll
     for (int ti = 0; ti < TPW; ti++) {
