@@ -49,6 +49,40 @@ __device__ inline void split_bf16x3(float x, __bf16 &x1, __bf16 &x2, __bf16 &x3)
 }
 constexpr int nsplit(int prec) { return prec == LMW_BF16X6 ? 3 : 2; }
 
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+// split_bf16x3 of two values at once, each part of the pair packed into one dword (a in the low half): one packed
+// conversion per part for both values, and the part's FP32 value recovered from the packed bits by a shift or a mask
+// (the same round-to-nearest-even parts as split_bf16x3; 11 instructions per pair instead of 15)
+__device__ inline void split3_pair(float a, float b, unsigned &p1, unsigned &p2, unsigned &p3) {
+#pragma clang fp contract(off)
+  p1 = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{a, b}, v2bf));
+  const float ra = a - __builtin_bit_cast(float, p1 << 16), rb = b - __builtin_bit_cast(float, p1 & 0xffff0000u);
+  p2 = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{ra, rb}, v2bf));
+  const float sa = ra - __builtin_bit_cast(float, p2 << 16), sb = rb - __builtin_bit_cast(float, p2 & 0xffff0000u);
+  p3 = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{sa, sb}, v2bf));
+}
+// split_bf16 (two parts) of two values at once, packed as in split3_pair
+__device__ inline void split2_pair(float a, float b, unsigned &p1, unsigned &p2) {
+#pragma clang fp contract(off)
+  p1 = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{a, b}, v2bf));
+  const float ra = a - __builtin_bit_cast(float, p1 << 16), rb = b - __builtin_bit_cast(float, p1 & 0xffff0000u);
+  p2 = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{ra, rb}, v2bf));
+}
+// eight FP32 values (lo = 0..3, hi = 4..7) split into their three BF16 parts as MFMA fragments
+__device__ inline void split8_regs(v4f lo, v4f hi, v8bf &a1, v8bf &a2, v8bf &a3) {
+  unsigned x[3][4];
+  split3_pair(lo[0], lo[1], x[0][0], x[1][0], x[2][0]);
+  split3_pair(lo[2], lo[3], x[0][1], x[1][1], x[2][1]);
+  split3_pair(hi[0], hi[1], x[0][2], x[1][2], x[2][2]);
+  split3_pair(hi[2], hi[3], x[0][3], x[1][3], x[2][3]);
+  const v4u u1{x[0][0], x[0][1], x[0][2], x[0][3]}, u2{x[1][0], x[1][1], x[1][2], x[1][3]}, u3{x[2][0], x[2][1], x[2][2], x[2][3]};
+  a1 = __builtin_bit_cast(v8bf, u1);
+  a2 = __builtin_bit_cast(v8bf, u2);
+  a3 = __builtin_bit_cast(v8bf, u3);
+}
+
 // acc += a * b for one 16x16 tile; PREC = LMW_FP32 consumes one float per lane, LMW_BF16X3 eight bf16 pairs
 __device__ inline v4f mma32(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 __device__ inline v4d mma64(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
@@ -72,15 +106,7 @@ __device__ inline v8bf load8(const __bf16 *p) { return *reinterpret_cast<const v
 // eight FP32 table entries (32-byte aligned) split into their three BF16 parts, as the tables are split for the
 // operands: 4 bytes per entry from memory instead of the 6 of three stored parts
 __device__ inline void split8(const float *p, v8bf &a1, v8bf &a2, v8bf &a3) {
-  const v4f lo = *reinterpret_cast<const v4f *>(p), hi = *reinterpret_cast<const v4f *>(p + 4);
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    __bf16 x1, x2, x3;
-    split_bf16x3(i < 4 ? lo[i] : hi[i - 4], x1, x2, x3);
-    a1[i] = x1;
-    a2[i] = x2;
-    a3[i] = x3;
-  }
+  split8_regs(*reinterpret_cast<const v4f *>(p), *reinterpret_cast<const v4f *>(p + 4), a1, a2, a3);
 }
 }  // namespace
 
@@ -249,13 +275,12 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
             cf[(q * RC + kk) * MEM + j] = v[0][q];
             cf[(q * RC + kk + 1) * MEM + j] = v[1][q];
           } else {
-            __bf16 h0, l0, t0, h1, l1, t1;
-            split_bf16x3(v[0][q], h0, l0, t0);
-            split_bf16x3(v[1][q], h1, l1, t1);
-            typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
-            *reinterpret_cast<v2bf *>(ch + (q * MEM + j) * SP + kk) = v2bf{h0, h1};
-            *reinterpret_cast<v2bf *>(cl + (q * MEM + j) * SP + kk) = v2bf{l0, l1};
-            if (PREC == LMW_BF16X6) *reinterpret_cast<v2bf *>(c3 + (q * MEM + j) * SP + kk) = v2bf{t0, t1};
+            unsigned p1, p2, p3;
+            if constexpr (PREC == LMW_BF16X6) split3_pair(v[0][q], v[1][q], p1, p2, p3);
+            else split2_pair(v[0][q], v[1][q], p1, p2);
+            *reinterpret_cast<unsigned *>(ch + (q * MEM + j) * SP + kk) = p1;
+            *reinterpret_cast<unsigned *>(cl + (q * MEM + j) * SP + kk) = p2;
+            if constexpr (PREC == LMW_BF16X6) *reinterpret_cast<unsigned *>(c3 + (q * MEM + j) * SP + kk) = p3;
           }
         }
       }
@@ -284,18 +309,8 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
           if constexpr (PREC == LMW_BF16X6) {
             v8bf aPh, aPl, aP3, aHh, aHl, aH3;
             if constexpr (PIPE) {
-#pragma unroll
-              for (int i = 0; i < 8; i++) {
-                __bf16 x1, x2, x3;
-                split_bf16x3(i < 4 ? araw[0][i] : araw[1][i - 4], x1, x2, x3);
-                aPh[i] = x1;
-                aPl[i] = x2;
-                aP3[i] = x3;
-                split_bf16x3(i < 4 ? araw[2][i] : araw[3][i - 4], x1, x2, x3);
-                aHh[i] = x1;
-                aHl[i] = x2;
-                aH3[i] = x3;
-              }
+              split8_regs(araw[0], araw[1], aPh, aPl, aP3);
+              split8_regs(araw[2], araw[3], aHh, aHl, aH3);
             } else {
               split8(T.Spf + base, aPh, aPl, aP3);
               split8(T.Shf + base, aHh, aHl, aH3);
@@ -476,9 +491,11 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
         // operands X (k = kk) and its H operands Y (k = kk + 1): parity 0 (even n - m) takes X from north + south
         // and Y from north - south, parity 1 the reverse (P(-mu) = (-1)^(n-m) P(mu), H(-mu) = -(-1)^(n-m) H(mu))
         const float sx = p ? -1.f : 1.f;
+        const unsigned keep = rr_ok[it] ? 0xffffffffu : 0u;  // zero the lanes whose clamped record is not used
+        auto kept = [&](float x) { return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) & keep); };
         float4 r[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) r[c] = rr_ok[it] ? rr[it][c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int c = 0; c < 5; c++) r[c] = make_float4(kept(rr[it][c].x), kept(rr[it][c].y), kept(rr[it][c].z), kept(rr[it][c].w));
         const float dm = static_cast<float>(m);
         // north: A r0.xy, B r0.zw, UP r1.xy, VP r1.zw, E r2.xy; south: A r2.zw, B r3.xy, UP r3.zw, VP r4.xy, E r4.zw
         const float Ax = r[0].x + sx * r[2].z, Ay = r[0].y + sx * r[2].w;  // X combination (P rows)
@@ -503,13 +520,12 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
           bf[(q * KC + kk) * MEM + j] = vv[it][0][q];
           bf[(q * KC + kk + 1) * MEM + j] = vv[it][1][q];
         } else {
-          __bf16 h0, l0, t0, h1, l1, t1;
-          split_bf16x3(vv[it][0][q], h0, l0, t0);
-          split_bf16x3(vv[it][1][q], h1, l1, t1);
-          typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
-          *reinterpret_cast<v2bf *>(bh + (q * MEM + j) * AP + kk) = v2bf{h0, h1};
-          *reinterpret_cast<v2bf *>(blo + (q * MEM + j) * AP + kk) = v2bf{l0, l1};
-          if (PREC == LMW_BF16X6) *reinterpret_cast<v2bf *>(b3p + (q * MEM + j) * AP + kk) = v2bf{t0, t1};
+          unsigned p1, p2, p3;
+          if constexpr (PREC == LMW_BF16X6) split3_pair(vv[it][0][q], vv[it][1][q], p1, p2, p3);
+          else split2_pair(vv[it][0][q], vv[it][1][q], p1, p2);
+          *reinterpret_cast<unsigned *>(bh + (q * MEM + j) * AP + kk) = p1;
+          *reinterpret_cast<unsigned *>(blo + (q * MEM + j) * AP + kk) = p2;
+          if constexpr (PREC == LMW_BF16X6) *reinterpret_cast<unsigned *>(b3p + (q * MEM + j) * AP + kk) = p3;
         }
       }
     }
@@ -522,12 +538,23 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       const int r = rbase + 16 * t + (lane & 15);
       const int kc = min(KC, KK - c0);  // KK is a multiple of 32
       if constexpr (PREC == LMW_FP64) {
-        for (int s = 0; s < kc / 4; s++) {
-          const int kk = 4 * s + (lane >> 4);
-          const double av = T.A64[static_cast<size_t>(KK) * G.poff[m] + static_cast<size_t>(c0 + kk) * kp + r];
+        // the tile's table operands of this chunk are requested FS k-steps at a time, so the MFMAs wait for one
+        // round trip per FS k-steps instead of one per k-step (kc == KC: KK is a multiple of 32)
+        constexpr int FS = TPW > 1 ? 4 : 8;  // (two tiles per wave leave registers for four)
+#pragma unroll 1
+        for (int s0 = 0; s0 < KC / 4; s0 += FS) {
+          double av[FS];
 #pragma unroll
-          for (int q = 0; q < 8; q++) accv[ti][q] = mma64(av, bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
+          for (int i = 0; i < FS; i++)
+            av[i] = T.A64[static_cast<size_t>(KK) * G.poff[m] + static_cast<size_t>(c0 + 4 * (s0 + i) + (lane >> 4)) * kp + r];
+#pragma unroll
+          for (int i = 0; i < FS; i++) {
+            const int kk = 4 * (s0 + i) + (lane >> 4);
+#pragma unroll
+            for (int q = 0; q < 8; q++) accv[ti][q] = mma64(av[i], bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
+          }
         }
+        (void)kc;
       } else if constexpr (PREC == LMW_FP32) {
         for (int s = 0; s < kc / 4; s++) {
           const int kk = 4 * s + (lane >> 4);
@@ -541,14 +568,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
           if constexpr (PREC == LMW_BF16X6) {
             v8bf ah, al, a3;
             if constexpr (PIPE) {
-#pragma unroll
-              for (int i = 0; i < 8; i++) {
-                __bf16 x1, x2, x3;
-                split_bf16x3(i < 4 ? araw[ti][0][i] : araw[ti][1][i - 4], x1, x2, x3);
-                ah[i] = x1;
-                al[i] = x2;
-                a3[i] = x3;
-              }
+              split8_regs(araw[ti][0], araw[ti][1], ah, al, a3);
             } else {
               split8(T.Abf + ai, ah, al, a3);
             }
@@ -667,15 +687,27 @@ __global__ __launch_bounds__(512) void k_synth_f64(Geo G, const double *__restri
       if (!active) continue;
       const int jh = jt * 16 + (lane & 15), rbase = G.poff[m] + (p ? kpe : 0);
       const int steps = min(RC, kp - c0) / 4;
-      for (int s = 0; s < steps; s++) {
-        const int kk = 4 * s + (lane >> 4);
-        const size_t ai = static_cast<size_t>(rbase + c0 + kk) * G.Jhp + jh;
-        const double aP = Ps[ai], aH = Hs[ai];
+      // the chunk's table operands are requested 8 k-steps at a time, so the MFMAs wait for one round trip per
+      // 8 k-steps instead of one per k-step (rows beyond the parity block load row 0 of it, unused)
+      for (int s0 = 0; s0 < steps; s0 += 8) {
+        double aP[8], aH[8];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const double bq = cs[q][kk][lane & 15];
-          accP[p][q] = mma64(aP, bq, accP[p][q]);
-          if (q >= 2) accH[p][q - 2] = mma64(aH, bq, accH[p][q - 2]);
+        for (int i = 0; i < 8; i++) {
+          const int kk = 4 * (s0 + i) + (lane >> 4);
+          const size_t ai = static_cast<size_t>(rbase + (s0 + i < steps ? c0 + kk : 0)) * G.Jhp + jh;
+          aP[i] = Ps[ai];
+          aH[i] = Hs[ai];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          if (s0 + i >= steps) break;
+          const int kk = 4 * (s0 + i) + (lane >> 4);
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const double bq = cs[q][kk][lane & 15];
+            accP[p][q] = mma64(aP[i], bq, accP[p][q]);
+            if (q >= 2) accH[p][q - 2] = mma64(aH[i], bq, accH[p][q - 2]);
+          }
         }
       }
     }

@@ -26,9 +26,10 @@ struct Geo {
   double a, g, omega;
   const int *moff, *poff, *kpe, *kpo;
   // element strides of the spectral state arrays x0/xs/acc: entry (member b, field f, packed q) at
-  // b sb + f sf + q sq.  Member-major [b][f][q] (sb = 3S, sf = S, sq = 1) for FP64 groups; member-minor [f][q][b]
-  // (sb = 1, sf = S B, sq = B) for mixed-precision groups of several members, whose Legendre kernels map lanes to
-  // members and so read and write 16 consecutive members as one contiguous 256-byte segment.
+  // b sb + f sf + q sq.  Member-major [b][f][q] (sb = 3S, sf = S, sq = 1) for a single member; member-minor [f][q][b]
+  // (sb = 1, sf = S B, sq = B) for groups of several members in every precision: the MFMA Legendre kernels map lanes
+  // to members and so read and write 8 or 16 consecutive members as one contiguous segment (the FP64 FMA kernels
+  // run on either layout).
   long long sb, sf, sq;
   __host__ __device__ size_t at(int b, int f, size_t q) const {
     return static_cast<size_t>(b * sb + f * sf + static_cast<long long>(q) * sq);

@@ -1149,8 +1149,8 @@ void SphereCore::set_precision(int p) {
   if (p != LMW_FP64 && p != LMW_FP32 && p != LMW_BF16X3 && p != LMW_BF16X6)
     invalid("precision must be LMW_FP64, LMW_FP32, LMW_BF16X3 or LMW_BF16X6");
   if (p != LMW_FP64) build_mixed_tables();
-  const bool minor = p != LMW_FP64 && B > 1;
-  if (minor != member_minor) {  // re-lay the state for the new precision's Legendre kernels
+  const bool minor = B > 1;  // groups of several members: member-minor in every precision (set at construction)
+  if (minor != member_minor) {
     const Geo Gs = geo();
     member_minor = minor;
     const Geo Gd = geo();
