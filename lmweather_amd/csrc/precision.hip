@@ -14,6 +14,7 @@
 // Each 16 x 16 MFMA tile holds one quantity for 16 members, so one lane's accumulators carry every quantity of
 // its member and the epilogues (north/south combination, RK4 update) need no data exchange.
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <vector>
 
@@ -790,15 +791,18 @@ struct SynthGroup {
 };
 struct MultiSynth {
   SynthGroup g[kMaxGroups];
-  const int4 *work;  // (group, m, member tile start, 0)
-  int nwork;
+  const int4 *work;   // (group, m, member tile start, first latitude tile)
+  const int *start;   // block x walks work[start[x] .. start[x + 1])
 };
+// blocks of kMultiWaves waves: an item covers kMultiWaves latitude tiles of one (m, member tile), as the per-group
+// kernel's latitude parts do
+constexpr int kMultiWaves = 4;
 template <int PREC, int RC>
-__global__ __launch_bounds__(512) void k_synth_mx_multi(MultiSynth M) {
-  for (int w = blockIdx.x; w < M.nwork; w += gridDim.x) {
+__global__ __launch_bounds__(64 * kMultiWaves, 3) void k_synth_mx_multi(MultiSynth M) {
+  for (int w = M.start[blockIdx.x]; w < M.start[blockIdx.x + 1]; w++) {
     const int4 it = M.work[w];
     const SynthGroup &g = M.g[it.x];
-    synth_mx_block<PREC, RC>(g.G, g.T, g.spec, g.Fs, 0, g.nb, 1, it.y, it.z);
+    synth_mx_block<PREC, RC>(g.G, g.T, g.spec, g.Fs, 0, g.nb, 1, it.y, it.z, it.w);
   }
 }
 
@@ -813,11 +817,12 @@ struct AnalGroup {
 struct MultiAnal {
   AnalGroup g[kMaxGroups];
   const int4 *work;  // (group, m, parity, member tile start)
-  int nwork, stage;
+  const int *start;  // block x walks work[start[x] .. start[x + 1])
+  int stage;
 };
 template <int PREC, int KC, int NW>
-__global__ __launch_bounds__(512) void k_anal_mx_multi(MultiAnal M) {
-  for (int w = blockIdx.x; w < M.nwork; w += gridDim.x) {
+__global__ __launch_bounds__(512, NW == 4 ? 3 : 1) void k_anal_mx_multi(MultiAnal M) {
+  for (int w = M.start[blockIdx.x]; w < M.start[blockIdx.x + 1]; w++) {
     const int4 it = M.work[w];
     const AnalGroup &g = M.g[it.x];
     anal_mx_block<PREC, KC, NW>(g.G, g.T, g.Gb, ANAL_TEND, M.stage, g.dt, g.nu4, g.x0, g.xs, g.acc, nullptr, 0, g.nb, 1, it.y, it.z, it.w);
@@ -967,66 +972,134 @@ void SphereCore::launch_anal_mx(int mode, int stage, double dt, int b0, int nb, 
 
 // ------------------------------------------------------------------------------------------- multi-group host side
 
+// resident blocks of `kernel` on the whole device
+template <class K> static int resident_blocks(K kernel, int threads, size_t lds) {
+  int per_cu = 0, dev = 0, cus = 0;
+  LMW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+  LMW_HIP(hipGetDevice(&dev));
+  LMW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(1, per_cu) * cus;
+}
+
+static size_t synth_multi_lds(int prec, int rc) {
+  return prec == LMW_FP32 ? 8 * rc * MEM * sizeof(float) : nsplit(prec) * 8 * (rc + 8) * MEM * sizeof(__bf16);
+}
+static size_t anal_multi_lds(int prec, int kc) {
+  return prec == LMW_FP32 ? 8 * kc * MEM * sizeof(float) : nsplit(prec) * 8 * (kc + 8) * MEM * sizeof(__bf16);
+}
+template <int PREC> static int synth_multi_blocks(int rc) {
+  const size_t lds = synth_multi_lds(PREC, rc);
+  return rc == 64 ? resident_blocks(k_synth_mx_multi<PREC, 64>, 64 * kMultiWaves, lds)
+                  : resident_blocks(k_synth_mx_multi<PREC, 32>, 64 * kMultiWaves, lds);
+}
+template <int PREC> static int anal_multi_blocks(int tile) {
+  const int kc = kAnalKC[tile], nw = kAnalNW[tile];
+  const size_t lds = anal_multi_lds(PREC, kc);
+  switch (tile) {
+    case 0: return resident_blocks(k_anal_mx_multi<PREC, 32, 4>, 64 * nw, lds);
+    case 2: return resident_blocks(k_anal_mx_multi<PREC, 128, 4>, 64 * nw, lds);
+    case 3: return resident_blocks(k_anal_mx_multi<PREC, 32, 8>, 64 * nw, lds);
+    case 4: return resident_blocks(k_anal_mx_multi<PREC, 64, 8>, 64 * nw, lds);
+    case 5: return resident_blocks(k_anal_mx_multi<PREC, 128, 8>, 64 * nw, lds);
+    default: return resident_blocks(k_anal_mx_multi<PREC, 64, 4>, 64 * nw, lds);
+  }
+}
+
+// Static schedule of one launch: the items of one key (same group and m, or 4 consecutive m for the inverse transform,
+// whose output records of 4 consecutive m share cache lines) go to one XCD -- block x runs on XCD x mod 8, the
+// hardware dealing blocks to the XCDs round robin -- so they share that XCD's L2 for the table rows and operands; keys
+// go to the least loaded XCD in decreasing order of cost, then each XCD's items, longest first, to its least loaded
+// block.  Costs estimate an item's duration from its chunk count (the K loop) and its epilogue.
+struct SchedItem {
+  int4 v;
+  int key;
+  double cost;
+};
+static void schedule(std::vector<SchedItem> items, int nblk, DevArray<int4> &work, DevArray<int> &start) {
+  std::map<int, double> key_cost;
+  for (const auto &it : items) key_cost[it.key] += it.cost;
+  std::vector<std::pair<double, int>> keys;
+  for (const auto &kc : key_cost) keys.push_back({kc.second, kc.first});
+  std::stable_sort(keys.begin(), keys.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+  const int nx = std::min(8, nblk);
+  std::vector<double> xload(nx, 0.0);
+  std::map<int, int> key_xcd;
+  for (const auto &k : keys) {
+    const int x = static_cast<int>(std::min_element(xload.begin(), xload.end()) - xload.begin());
+    xload[x] += k.first;
+    key_xcd[k.second] = x;
+  }
+  std::stable_sort(items.begin(), items.end(), [](const SchedItem &a, const SchedItem &b) { return a.cost > b.cost; });
+  std::vector<std::vector<int4>> lists(nblk);
+  std::vector<double> bload(nblk, 0.0);
+  for (const auto &it : items) {
+    const int x = key_xcd[it.key];
+    int best = -1;
+    for (int blk = x; blk < nblk; blk += nx)
+      if (best < 0 || bload[blk] < bload[best]) best = blk;
+    bload[best] += it.cost;
+    lists[best].push_back(it.v);
+  }
+  std::vector<int4> flat;
+  std::vector<int> off(nblk + 1, 0);
+  for (int blk = 0; blk < nblk; blk++) {
+    off[blk] = static_cast<int>(flat.size());
+    flat.insert(flat.end(), lists[blk].begin(), lists[blk].end());
+  }
+  off[nblk] = static_cast<int>(flat.size());
+  work.alloc(std::max<size_t>(flat.size(), 1));
+  start.alloc(nblk + 1);
+  if (!flat.empty()) LMW_HIP(hipMemcpy(work.p, flat.data(), flat.size() * sizeof(int4), hipMemcpyHostToDevice));
+  LMW_HIP(hipMemcpy(start.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
 void build_multi_work(const std::vector<SphereCore *> &groups, unsigned active_mask, MultiWork &w) {
-  struct Item {
-    int4 v;
-    double cost;
-  };
-  std::vector<Item> sy, an;
+  const SphereCore &big = *groups.back();
+  const int rc = kSynthRC[big.tile_synth], kc = kAnalKC[big.tile_anal], nw = kAnalNW[big.tile_anal];
+  switch (big.precision) {
+    case LMW_FP32: w.synth_blocks = synth_multi_blocks<LMW_FP32>(rc); w.anal_blocks = anal_multi_blocks<LMW_FP32>(big.tile_anal); break;
+    case LMW_BF16X6: w.synth_blocks = synth_multi_blocks<LMW_BF16X6>(rc); w.anal_blocks = anal_multi_blocks<LMW_BF16X6>(big.tile_anal); break;
+    default: w.synth_blocks = synth_multi_blocks<LMW_BF16X3>(rc); w.anal_blocks = anal_multi_blocks<LMW_BF16X3>(big.tile_anal); break;
+  }
+  std::vector<SchedItem> sy, an;
   for (int g = 0; g < static_cast<int>(groups.size()); g++) {
     if (!(active_mask & (1u << g))) continue;
     const SphereCore &c = *groups[g];
-    for (int m = 0; m <= c.N; m++)
+    const int lat_tiles = (c.Jh + 15) / 16, parts = (lat_tiles + kMultiWaves - 1) / kMultiWaves;
+    for (int m = 0; m <= c.N; m++) {
+      const int Km = c.N - m + 1;
+      const int ke32 = ((Km + 1) / 2 + 31) / 32 * 32, ko32 = (Km / 2 + 31) / 32 * 32;
+      const int chunks = (ke32 + rc - 1) / rc + (ko32 + rc - 1) / rc;
       for (int mb = 0; mb < c.B; mb += MEM) {
-        const int Km = c.N - m + 1;
-        sy.push_back({make_int4(g, m, mb, 0), static_cast<double>(Km) * c.Jh});
+        for (int pt = 0; pt < parts; pt++)
+          sy.push_back({make_int4(g, m, mb, pt * kMultiWaves), g * 4096 + m / 4, 4.6 * chunks * (rc / 32) + 3.0});
         for (int p = 0; p < 2; p++) {
           const int rows = p ? c.h_kpo[m] : c.h_kpe[m];
-          if (rows > 0) an.push_back({make_int4(g, m, p, mb), static_cast<double>(rows) * 2 * c.Jhp});
+          if (rows == 0) continue;
+          const int tpw = (rows / 16 + nw - 1) / nw;  // row tiles of the busiest wave
+          an.push_back({make_int4(g, m, p, mb), g * 4096 + m, 2.9 * (2 * c.Jhp / kc) * (kc / 32) + 6.4 * tpw + 2.0});
         }
       }
+    }
   }
-  auto by_cost = [](const Item &a, const Item &b) { return a.cost > b.cost; };
-  std::stable_sort(sy.begin(), sy.end(), by_cost);
-  std::stable_sort(an.begin(), an.end(), by_cost);
-  std::vector<int4> vs(sy.size()), va(an.size());
-  for (size_t i = 0; i < sy.size(); i++) vs[i] = sy[i].v;
-  for (size_t i = 0; i < an.size(); i++) va[i] = an[i].v;
-  w.synth.alloc(std::max<size_t>(vs.size(), 1));
-  w.anal.alloc(std::max<size_t>(va.size(), 1));
-  if (!vs.empty()) LMW_HIP(hipMemcpy(w.synth.p, vs.data(), vs.size() * sizeof(int4), hipMemcpyHostToDevice));
-  if (!va.empty()) LMW_HIP(hipMemcpy(w.anal.p, va.data(), va.size() * sizeof(int4), hipMemcpyHostToDevice));
-  w.nsynth = static_cast<int>(vs.size());
-  w.nanal = static_cast<int>(va.size());
-}
-
-// resident blocks of `kernel` on the whole device, at most `want`
-template <class K> static int persistent_blocks(K kernel, int threads, size_t lds, int want) {
-  int per_cu = 0, dev = 0;
-  LMW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
-  LMW_HIP(hipGetDevice(&dev));
-  int cus = 0;
-  LMW_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  return std::max(1, std::min(want, std::max(1, per_cu) * cus));
+  w.nsynth = static_cast<int>(sy.size());
+  w.nanal = static_cast<int>(an.size());
+  schedule(sy, w.synth_blocks, w.synth, w.synth_start);
+  schedule(an, w.anal_blocks, w.anal, w.anal_start);
 }
 
 template <int PREC, int RC>
 static void synth_multi(const std::vector<SphereCore *> &gs, const MultiWork &w, hipStream_t stream) {
   MultiSynth M{};
-  int waves = 1;
   for (size_t g = 0; g < gs.size(); g++) {
     SphereCore &c = *gs[g];
     M.g[g] = SynthGroup{c.geo(), SynthTabs{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Spf.p, c.Shf.p, c.d_soff.p, c.d_ke32.p,
                                              c.d_ko32.p},
                         c.xs.p, c.Fs32.p, c.B};
-    waves = std::max(waves, (c.Jh + 15) / 16);
   }
   M.work = w.synth.p;
-  M.nwork = w.nsynth;
-  const int threads = 64 * waves;
-  const size_t lds = PREC == LMW_FP32 ? 8 * RC * MEM * sizeof(float) : nsplit(PREC) * 8 * (RC + 8) * MEM * sizeof(__bf16);
-  const int nblk = persistent_blocks(k_synth_mx_multi<PREC, RC>, threads, lds, w.nsynth);
-  hipLaunchKernelGGL((k_synth_mx_multi<PREC, RC>), dim3(nblk), dim3(threads), lds, stream, M);
+  M.start = w.synth_start.p;
+  hipLaunchKernelGGL((k_synth_mx_multi<PREC, RC>), dim3(w.synth_blocks), dim3(64 * kMultiWaves), synth_multi_lds(PREC, RC), stream, M);
   LMW_LAUNCH_CHECK();
 }
 
@@ -1050,11 +1123,9 @@ static void anal_multi(const std::vector<SphereCore *> &gs, const std::vector<do
                        c.B};
   }
   M.work = w.anal.p;
-  M.nwork = w.nanal;
+  M.start = w.anal_start.p;
   M.stage = stage;
-  const size_t lds = PREC == LMW_FP32 ? 8 * KC * MEM * sizeof(float) : nsplit(PREC) * 8 * (KC + 8) * MEM * sizeof(__bf16);
-  const int nblk = persistent_blocks(k_anal_mx_multi<PREC, KC, NW>, 64 * NW, lds, w.nanal);
-  hipLaunchKernelGGL((k_anal_mx_multi<PREC, KC, NW>), dim3(nblk), dim3(64 * NW), lds, stream, M);
+  hipLaunchKernelGGL((k_anal_mx_multi<PREC, KC, NW>), dim3(w.anal_blocks), dim3(64 * NW), anal_multi_lds(PREC, KC), stream, M);
   LMW_LAUNCH_CHECK();
 }
 

@@ -199,10 +199,13 @@ struct SphereCore {
 
 // Persistent multi-group Legendre launches (precision.hip): groups[i] are all the groups of an ensemble (same mixed
 // precision, at most kMaxGroups); work items name them by index; dts[i] is group i's step.  Tiles: those of the last
-// (largest) group.
+// (largest) group.  Each launch has as many blocks as fit on the device at once; block x walks its own list of items,
+// items synth/anal[start[x] .. start[x + 1]), built on the host (build_multi_work).
 struct MultiWork {
-  DevArray<int4> synth, anal;  // (group, m, member tile start, 0) / (group, m, parity, member tile start)
-  int nsynth = 0, nanal = 0;
+  // (group, m, member tile start, first latitude tile) / (group, m, parity, member tile start)
+  DevArray<int4> synth, anal;
+  DevArray<int> synth_start, anal_start;
+  int nsynth = 0, nanal = 0, synth_blocks = 0, anal_blocks = 0;
 };
 void build_multi_work(const std::vector<SphereCore *> &groups, unsigned active_mask, MultiWork &w);
 void launch_synth_mx_multi(const std::vector<SphereCore *> &groups, const MultiWork &w, hipStream_t stream);
