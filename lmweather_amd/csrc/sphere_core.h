@@ -90,6 +90,7 @@ struct SphereCore {
   DevArray<__bf16> Abh, Abl, Sph, Spl, Shh, Shl;  // BF16 parts 1, 2 (BF16X3)
   DevArray<float> Abf, Spf, Shf;                  // the same FP32 values in that layout, split in three on load (BF16X6)
   DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K
+  DevArray<float2> tw32;                       // tw rounded to FP32 (mixed-precision grid kernel)
   DevArray<double2> x0, xs, acc;               // [b][3][S]
   DevArray<double2> Fs;                        // [b][J][M1][4]
   DevArray<double2> Gb;                        // [b][M1][Jh][16]

@@ -66,6 +66,13 @@ template <class F, class C> __device__ inline C rfft_coef(const C *arr, int m, c
   const C Fo = C{half * (Zm.y + Zc.y), half * (Zc.x - Zm.x)};  // (Zm - conj Zc) / 2i: odd samples
   return cscale(cadd(Fe, cmul(tw[m], Fo)), invK);
 }
+// 2 K rfft_coef, formed without the halvings: (Zm + conj Zc) + tw[m] (Zm - conj Zc) / i.  Each scaling by a power of two
+// is exact, so X scale(2 K rfft_coef) rounds once like rfft_coef times the same weight: scale = weight / (2K).
+template <class F, class C> __device__ inline C rfft_coef2K(const C *arr, int m, const C *tw) {
+  const C Zm = arr[F::phys(F::pos(m))], Zc = arr[F::phys(F::pos((F::L - m) & (F::L - 1)))];
+  const C Se = C{Zm.x + Zc.x, Zm.y - Zc.y}, So = C{Zm.y + Zc.y, Zc.x - Zm.x};
+  return cadd(Se, cmul(tw[m], So));
+}
 
 // Backward: entry m' (0 <= m' < L) of the half-length spectrum whose backward transform is x[2k] + i x[2k+1] for
 // the real row x_k = sum_m a_m exp(2 pi i m k / K) with a_0 = Re F_0, a_m = F_m, a_{K-m} = conj F_m (m = 1..N) and
@@ -316,7 +323,7 @@ __global__ __launch_bounds__(64 * MB) void k_synth(Geo G, const double *__restri
 // (double2: the FP64 step; float2: the mixed-precision step, whose rows hold FP32 values already).
 template <int R1, int R2, class TS>
 __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__ Fs, const double *__restrict__ fgrid,
-                                                   const double *__restrict__ hs, const double2 *__restrict__ twg,
+                                                   const double *__restrict__ hs, const TS *__restrict__ twg,
                                                    const double *__restrict__ mu, const double *__restrict__ w,
                                                    TS *__restrict__ Gb, int b0, const int *__restrict__ mflags) {
   using F = Fft2<R1, R2>;
@@ -432,21 +439,26 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   fft2<R1, R2, -1>(buf, 10, tw);
   const T wj = static_cast<T>(w[jn]), wd = static_cast<T>(w[jn] / (1 - mu[jn] * mu[jn]) / G.a), invK = static_cast<T>(1.0 / K);
   for (int m = tid; m < M1; m += blockDim.x) {
-    C c[10];
-#pragma unroll
-    for (int r = 0; r < 10; r++) c[r] = rfft_coef<F>(buf + r * P, m, tw, invK);
     if constexpr (sizeof(TS) == 8) {
       // FP32: one 80-byte record per (member, m, latitude pair) of the five Fourier coefficients at both latitudes,
       // weighted (wd A, wd B, wd U phi, wd V phi, wj E; north then south); the forward transform forms the i m
       // factors and the north/south combinations of each row parity as it loads them (five stored values per
-      // latitude instead of eight operands per parity)
+      // latitude instead of eight operands per parity).  The weights absorb 1 / (2K): the same bits as the weight
+      // times rfft_coef.
+      const T wd2 = wd * (invK * T(0.5)), wj2 = wj * (invK * T(0.5));
+      C c[10];
+#pragma unroll
+      for (int r = 0; r < 10; r++) c[r] = rfft_coef2K<F>(buf + r * P, m, tw);
       float4 *o = reinterpret_cast<float4 *>(Gb + ((static_cast<size_t>(b) * M1 + m) * G.Jh + jh) * 10);
-      const C v[10] = {cscale(c[0], wd), cscale(c[2], wd), cscale(c[4], wd), cscale(c[6], wd), cscale(c[8], wj),
-                       cscale(c[1], wd), cscale(c[3], wd), cscale(c[5], wd), cscale(c[7], wd), cscale(c[9], wj)};
+      const C v[10] = {cscale(c[0], wd2), cscale(c[2], wd2), cscale(c[4], wd2), cscale(c[6], wd2), cscale(c[8], wj2),
+                       cscale(c[1], wd2), cscale(c[3], wd2), cscale(c[5], wd2), cscale(c[7], wd2), cscale(c[9], wj2)};
 #pragma unroll
       for (int i = 0; i < 5; i++) o[i] = make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
       continue;
     }
+    C c[10];
+#pragma unroll
+    for (int r = 0; r < 10; r++) c[r] = rfft_coef<F>(buf + r * P, m, tw, invK);
     const T dm = static_cast<T>(m);
     // dzeta -= (i m A wd) P - (B wd) H; ddelta += (i m B wd) P + (A wd) H + (E w) P n(n+1)/a^2;
     // dphi -= (i m U phi wd) P - (V phi wd) H   (sphere.cpp:374-383); index 0 north, 1 south
@@ -901,6 +913,11 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
   }
   tw.alloc(K);
   tw.upload(htw.data(), K, 0, stream);
+  std::vector<float2> htw32(K);  // the same twiddles rounded once to FP32, for the mixed-precision grid kernel
+  for (int q = 0; q < K; q++) htw32[q] = make_float2(static_cast<float>(htw[q].x), static_cast<float>(htw[q].y));
+  tw32.alloc(K);
+  tw32.upload(htw32.data(), K, 0, stream);
+  sync();
   const size_t nstate = static_cast<size_t>(B) * 3 * S;
   x0.alloc(nstate);
   xs.alloc(nstate);
@@ -1105,8 +1122,8 @@ static size_t grid_lds_bytes(int K) {
 
 void SphereCore::launch_tend_grid(int b0, int nb, int mixed) {
   if (mixed) {  // FP32 rows, FP32 arithmetic: half the LDS of the FP64 kernel
-    LMW_DISPATCH_LT(K / 2, k_tend_grid, float2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K) / 2, stream, geo(), Fs32.p, f.p, hs.p, tw.p, mu.p,
-                    w.p, Gb32.p, b0, mflags.p)
+    LMW_DISPATCH_LT(K / 2, k_tend_grid, float2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K) / 2, stream, geo(), Fs32.p, f.p, hs.p, tw32.p,
+                    mu.p, w.p, Gb32.p, b0, mflags.p)
   } else {
     LMW_DISPATCH_LT(K / 2, k_tend_grid, double2, dim3(Jh, nb), dim3(256), grid_lds_bytes(K), stream, geo(), Fs.p, f.p, hs.p, tw.p, mu.p, w.p,
                     Gb.p, b0, mflags.p)
