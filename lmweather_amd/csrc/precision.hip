@@ -191,6 +191,7 @@ struct SynthTabs {
   const __bf16 *Sph, *Spl, *Shh, *Shl;
   const float *Spf, *Shf;  // FP32 in the BF16 layout, split into three parts on load (BF16X6)
   const int *soff, *ke32, *ko32;
+  const double *kn;  // kn[n] = -a^2 / (n (n + 1)) for 1 <= n <= N + 1, kn[0] = 0: psi, chi = kn (zeta, delta)
 };
 
 // Block: one m, 16 members, one wave per 16 northern latitudes.  Coefficients are staged in LDS in chunks of 32
@@ -209,6 +210,7 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
   __bf16 *c3 = cl + 8 * MEM * SP;                                           // part 3 (BF16X6)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int Km = G.N - m + 1, kpe = G.kpe[m], kpo = G.kpo[m];
+  const size_t moff_m = static_cast<size_t>(G.moff[m]);
   const double a = G.a;
   const int jt = jt0 + wave;  // row tile (16 northern latitudes) of this wave
   const bool active = jt * 16 < G.Jh;
@@ -245,30 +247,40 @@ __device__ __forceinline__ void synth_mx_block(const Geo &G, const SynthTabs &T,
         }
       }
       // stage 8 quantities x RC rows x 16 members: one thread per (member, pair of rows), members fastest, so
-      // that a wavefront writes consecutive LDS words (FP32) or whole words of a padded pitch (BF16)
+      // that a wavefront writes consecutive LDS words (FP32) or whole words of a padded pitch (BF16).  Both rows'
+      // coefficients (and their kn) are loaded before either is used, from a clamped valid row and member where the
+      // lane has none (zeroed afterwards), so the thread waits for memory once per chunk.
       for (int e = threadIdx.x; e < (RC / 2) * MEM; e += blockDim.x) {
         const int j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
+        const int bc = b0 + min(bl, nb - 1);
+        double2 z[2], d[2], f[2];
+        double knv[2];
+        bool ok[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int kq = c0 + kk + h, k = 2 * kq + p;
+          ok[h] = bl < nb && k < Km && kq < kp;
+          const size_t q = moff_m + min(k, Km - 1);
+          z[h] = spec[G.at(bc, 0, q)];
+          d[h] = spec[G.at(bc, 1, q)];
+          f[h] = spec[G.at(bc, 2, q)];
+          knv[h] = T.kn[min(m + k, G.N + 1)];
+        }
         float v[2][8];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-          const int kq = c0 + kk + h, k = 2 * kq + p, n = m + k;
-          double2 z = cx(0, 0), d = cx(0, 0), f = cx(0, 0);
-          if (bl < nb && k < Km && kq < kp) {
-            const size_t q = static_cast<size_t>(G.moff[m]) + k;
-            z = spec[G.at(b0 + bl, 0, q)];
-            d = spec[G.at(b0 + bl, 1, q)];
-            f = spec[G.at(b0 + bl, 2, q)];
-            if (mixed && n == 0) f = cx(0, 0);  // phi_00 P_0^0 (the area mean) stays out of the mixed path
-          }
-          const double kn = n >= 1 ? -a * a / (n * (n + 1.0)) : 0.0;  // psi, chi = kn (zeta, delta): sphere.cpp:185-190
-          v[h][0] = static_cast<float>(z.x);
-          v[h][1] = static_cast<float>(z.y);
-          v[h][2] = static_cast<float>(f.x);
-          v[h][3] = static_cast<float>(f.y);
-          v[h][4] = static_cast<float>(z.x * kn);
-          v[h][5] = static_cast<float>(z.y * kn);
-          v[h][6] = static_cast<float>(d.x * kn);
-          v[h][7] = static_cast<float>(d.y * kn);
+          const int n = m + 2 * (c0 + kk + h) + p;
+          if (!ok[h]) z[h] = d[h] = f[h] = cx(0, 0);
+          if (mixed && n == 0) f[h] = cx(0, 0);  // phi_00 P_0^0 (the area mean) stays out of the mixed path
+          const double kn = knv[h];  // psi, chi = kn (zeta, delta): sphere.cpp:185-190
+          v[h][0] = static_cast<float>(z[h].x);
+          v[h][1] = static_cast<float>(z[h].y);
+          v[h][2] = static_cast<float>(f[h].x);
+          v[h][3] = static_cast<float>(f[h].y);
+          v[h][4] = static_cast<float>(z[h].x * kn);
+          v[h][5] = static_cast<float>(z[h].y * kn);
+          v[h][6] = static_cast<float>(d[h].x * kn);
+          v[h][7] = static_cast<float>(d[h].y * kn);
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) {
@@ -883,7 +895,7 @@ const int kAnalNW[kAnalTiles] = {4, 4, 4, 8, 8, 8};
 
 template <int PREC, int RC>
 static void synth_mx_launch(SphereCore &c, const double2 *spec, int b0, int nb) {
-  SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Spf.p, c.Shf.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p};
+  SynthTabs T{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Spf.p, c.Shf.p, c.d_soff.p, c.d_ke32.p, c.d_ko32.p, c.knt.p};
   const int waves = ((c.Jh + 15) / 16 + kSynthJSplit - 1) / kSynthJSplit;  // row tiles of one latitude part
   const int tiles = (nb + MEM - 1) / MEM;
   dim3 grd(((c.M1 + 3) / 4 + 7) / 8 * 8 * 4 * tiles * kSynthJSplit), blk(64 * waves);
@@ -1094,7 +1106,7 @@ static void synth_multi(const std::vector<SphereCore *> &gs, const MultiWork &w,
   for (size_t g = 0; g < gs.size(); g++) {
     SphereCore &c = *gs[g];
     M.g[g] = SynthGroup{c.geo(), SynthTabs{c.Ps32.p, c.Hs32.p, c.Sph.p, c.Spl.p, c.Shh.p, c.Shl.p, c.Spf.p, c.Shf.p, c.d_soff.p, c.d_ke32.p,
-                                             c.d_ko32.p},
+                                             c.d_ko32.p, c.knt.p},
                         c.xs.p, c.Fs32.p, c.B};
   }
   M.work = w.synth.p;

@@ -91,6 +91,7 @@ struct SphereCore {
   DevArray<float> Abf, Spf, Shf;                  // the same FP32 values in that layout, split in three on load (BF16X6)
   DevArray<double2> tw;                        // exp(-2 pi i q / K), q < K
   DevArray<float2> tw32;                       // tw rounded to FP32 (mixed-precision grid kernel)
+  DevArray<double> knt;                        // -a^2 / (n (n + 1)), n = 0 .. N + 1 (0 at n = 0): psi, chi from zeta, delta
   DevArray<double2> x0, xs, acc;               // [b][3][S]
   DevArray<double2> Fs;                        // [b][J][M1][4]
   DevArray<double2> Gb;                        // [b][M1][Jh][16]

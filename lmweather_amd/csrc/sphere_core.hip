@@ -345,18 +345,24 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   const int flags = mflags[b];
   const bool untilted = flags & 1, flat = flags & 2;
   const size_t hplane = static_cast<size_t>(flags >> 2);  // this member's orography plane
-  // orography and f at this thread's longitudes of both rows
-  T hsv[2][2], fv[2][2];
+  // Every global load of the prologue is issued before the first value is used, from a clamped valid address where
+  // a lane has nothing to load (its value is zeroed afterwards), so the loads are in flight together and the thread
+  // waits once: orography and f of this thread's longitudes of both rows (skipped for the whole block when the
+  // member has no orography or an untilted axis), the four fields' coefficients at m' and L - m' of each inverse-FFT
+  // input, the twiddles.  Converted from FP64 only after the coefficients have arrived.
+  const int kc = k < L ? k : L - 1;
+  double hraw[2][2] = {{0, 0}, {0, 0}}, fraw[2][2] = {{0, 0}, {0, 0}};
+  if (!flat) {
 #pragma unroll
-  for (int s = 0; s < 2; s++) {
-    const size_t gi = (static_cast<size_t>(b) * G.J + (s ? js : jn)) * K + 2 * k;
-    const size_t gh = (hplane * G.J + (s ? js : jn)) * K + 2 * k;
+    for (int s = 0; s < 2; s++)
 #pragma unroll
-    for (int e = 0; e < 2; e++) {
-      hsv[s][e] = fv[s][e] = T(0);
-      if (k < L && !flat) hsv[s][e] = static_cast<T>(hs[gh + e]);
-      if (k < L && !untilted) fv[s][e] = static_cast<T>(fgrid[gi + e]);
-    }
+      for (int e = 0; e < 2; e++) hraw[s][e] = hs[(hplane * G.J + (s ? js : jn)) * K + 2 * kc + e];
+  }
+  if (!untilted) {
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int e = 0; e < 2; e++) fraw[s][e] = fgrid[(static_cast<size_t>(b) * G.J + (s ? js : jn)) * K + 2 * kc + e];
   }
   // inverse real FFT inputs (the arithmetic of irfft_entry): per (row s, m') the four fields' coefficients at m'
   // and L - m', 4 contiguous complex values each
@@ -365,39 +371,40 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
     C twm[NIT];
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
-      const int it = tid + 256 * q, s = it / L, mp = it - s * L;
+      const int it = min(tid + 256 * q, 2 * L - 1), s = it / L, mp = it - s * L;
+      const TS *row = Fs + (static_cast<size_t>(b) * G.J + (s ? js : jn)) * M1 * 4;
+      const int ml = min(mp, G.N), mh = min(L - mp, G.N);  // (clamped: lanes without a coefficient read a valid one)
 #pragma unroll
-      for (int f = 0; f < 4; f++) lo[q][f] = hi[q][f] = TS{0, 0};
-      twm[q] = mkc<C>(0, 0);
-      if (it < 2 * L) {
-        const TS *row = Fs + (static_cast<size_t>(b) * G.J + (s ? js : jn)) * M1 * 4;
-        if (mp <= G.N) {
+      for (int f = 0; f < 4; f++) lo[q][f] = row[4 * ml + f];
 #pragma unroll
-          for (int f = 0; f < 4; f++) lo[q][f] = row[4 * mp + f];
-        }
-        if (mp >= 1 && L - mp <= G.N) {
-#pragma unroll
-          for (int f = 0; f < 4; f++) hi[q][f] = row[4 * (L - mp) + f];
-        }
-        twm[q] = to_c<C>(twg[mp]);
-      }
+      for (int f = 0; f < 4; f++) hi[q][f] = row[4 * mh + f];
+      twm[q] = to_c<C>(twg[mp]);
     }
     for (int i = tid; i < K; i += blockDim.x) tw[i] = to_c<C>(twg[i]);
 #pragma unroll
     for (int q = 0; q < NIT; q++) {
       const int it = tid + 256 * q, s = it / L, mp = it - s * L;
       if (it < 2 * L) {
+        const bool has_lo = mp <= G.N, has_hi = mp >= 1 && L - mp <= G.N;
 #pragma unroll
         for (int f = 0; f < 4; f++) {
-          C a = to_c<C>(lo[q][f]);
+          C a = has_lo ? to_c<C>(lo[q][f]) : mkc<C>(0, 0);
           if (mp == 0) a.y = 0;
-          const C c = mp >= 1 && L - mp <= G.N ? cconj(to_c<C>(hi[q][f])) : mkc<C>(0, 0);
+          const C c = has_hi ? cconj(to_c<C>(hi[q][f])) : mkc<C>(0, 0);
           const C E = cadd(a, c), O = cmul(csub(a, c), cconj(twm[q]));
           buf[(2 * f + s) * P + F::phys(mp)] = C{E.x - O.y, E.y + O.x};  // E + i O
         }
       }
     }
   }
+  T hsv[2][2], fv[2][2];
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      hsv[s][e] = k < L ? static_cast<T>(hraw[s][e]) : T(0);
+      fv[s][e] = k < L ? static_cast<T>(fraw[s][e]) : T(0);
+    }
   __syncthreads();
   fft2<R1, R2, +1>(buf, 8, tw);
   // mixed precision: the phi rows hold phi' = phi - phibar (phibar = phi_00 / sqrt 2, the area mean), so the phi
@@ -917,6 +924,10 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
   for (int q = 0; q < K; q++) htw32[q] = make_float2(static_cast<float>(htw[q].x), static_cast<float>(htw[q].y));
   tw32.alloc(K);
   tw32.upload(htw32.data(), K, 0, stream);
+  std::vector<double> hkn(N + 2);  // the inverse Laplacian of the synthesis kernels (sphere.cpp:185-190), same expression
+  for (int n = 0; n <= N + 1; n++) hkn[n] = n >= 1 ? -prm.radius * prm.radius / (n * (n + 1.0)) : 0.0;
+  knt.alloc(N + 2);
+  knt.upload(hkn.data(), N + 2, 0, stream);
   sync();
   const size_t nstate = static_cast<size_t>(B) * 3 * S;
   x0.alloc(nstate);
