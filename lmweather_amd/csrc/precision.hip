@@ -425,6 +425,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Rp = p ? G.kpo[m] : kpe, rbase = p ? kpe : 0;
   if (Rp == 0) return;
   const int nrt = Rp / 16, KK = 2 * G.Jhp;
+  const size_t moff_m = static_cast<size_t>(G.moff[m]);  // read once, before the epilogue's stores
   AV accv[TPW][8];
   for (int t = 0; t < TPW; t++)
     for (int q = 0; q < 8; q++) accv[t][q] = AV{0, 0, 0, 0};
@@ -621,19 +622,20 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   auto elem_ok = [&](int e) { return wave + NW * (e >> 2) < nrt && elem_k(e) < Km; };
   if (mode == ANAL_TEND) {
     // the RK4 update with the state loads of element e + 1 issued before the arithmetic and stores of element e
-    const double phibar = mixed ? xs[G.at(b, 2, 0)].x * 0.70710678118654752440 : 0.0;  // phi_00 P_0^0
+    // phi_00 P_0^0 of the stage input, requested with the first element's state
+    const double phi00 = mixed ? xs[G.at(b, 2, 0)].x : 0.0;
     TendIn next = {};
-    if (elem_ok(0)) next = tend_load(G, stage, b, static_cast<size_t>(G.moff[m]) + elem_k(0), x0, xs, acc, mixed);
+    if (elem_ok(0)) next = tend_load(G, stage, b, moff_m + elem_k(0), x0, xs, acc, mixed);
+    const double phibar = phi00 * 0.70710678118654752440;
 #pragma unroll
     for (int e = 0; e < 4 * TPW; e++) {
       const TendIn cur = next;
-      if (e + 1 < 4 * TPW && elem_ok(e + 1))
-        next = tend_load(G, stage, b, static_cast<size_t>(G.moff[m]) + elem_k(e + 1), x0, xs, acc, mixed);
+      if (e + 1 < 4 * TPW && elem_ok(e + 1)) next = tend_load(G, stage, b, moff_m + elem_k(e + 1), x0, xs, acc, mixed);
       if (!elem_ok(e)) continue;
       const int ti = e >> 2, r = e & 3;
       const double2 o1 = cx(accv[ti][0][r], accv[ti][1][r]), o2 = cx(accv[ti][2][r], accv[ti][3][r]);
       const double2 o3 = cx(accv[ti][4][r], accv[ti][5][r]), o4 = cx(accv[ti][6][r], accv[ti][7][r]);
-      tend_store(G, stage, dt, nu4, b, m, elem_k(e), o1, o2, o3, o4, cur, phibar, x0, xs, acc, mixed);
+      tend_store(G, stage, dt, nu4, b, m + elem_k(e), moff_m + elem_k(e), o1, o2, o3, o4, cur, phibar, x0, xs, acc, mixed);
     }
     return;
   }
@@ -643,7 +645,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
     const int ti = e >> 2, r = e & 3;
     const double2 o1 = cx(accv[ti][0][r], accv[ti][1][r]), o2 = cx(accv[ti][2][r], accv[ti][3][r]);
     const double2 o3 = cx(accv[ti][4][r], accv[ti][5][r]), o4 = cx(accv[ti][6][r], accv[ti][7][r]);
-    anal_epilogue(G, mode, stage, dt, nu4, b, bl, m, elem_k(e), o1, o2, o3, o4, x0, xs, acc, raw, mixed);
+    anal_epilogue(G, mode, stage, dt, nu4, b, bl, m + elem_k(e), moff_m + elem_k(e), o1, o2, o3, o4, x0, xs, acc, raw, mixed);
   }
 }
 

@@ -566,6 +566,7 @@ __global__ __launch_bounds__(64 * MB) void k_anal(Geo G, const double *__restric
   const int r = blockIdx.y * 64 + tx;
   const int bl = blockIdx.z * MB + ty, b = b0 + bl;
   const int kpe = G.kpe[m], kp = kpe + G.kpo[m], Km = G.N - m + 1;
+  const size_t q = static_cast<size_t>(G.moff[m]);  // + k below
   const bool odd = r >= kpe;
   const int k = odd ? 2 * (r - kpe) + 1 : 2 * r;
   const bool valid = (bl < nb) && (r < kp) && (k < Km);
@@ -603,7 +604,7 @@ __global__ __launch_bounds__(64 * MB) void k_anal(Geo G, const double *__restric
     }
   }
   if (!valid) return;
-  anal_epilogue(G, mode, stage, dt, nu4, b, bl, m, k, o1, o2, o3, o4, x0, xs, acc, raw, 0);
+  anal_epilogue(G, mode, stage, dt, nu4, b, bl, m + k, q + k, o1, o2, o3, o4, x0, xs, acc, raw, 0);
 }
 
 // Inverse FFTs of the four Fourier fields to the grid: diagnostics (h = phi/g, u = U/cos, v = V/cos, zeta;

@@ -25,11 +25,11 @@ __device__ inline TendIn tend_load(const Geo &G, int stage, int b, size_t q, con
   t.d = mixed ? xs[G.at(b, 1, q)] : cx(0, 0);
   return t;
 }
-__device__ inline void tend_store(const Geo &G, int stage, double dt, double nu4, int b, int m, int k, double2 o1, double2 o2, double2 o3,
+// (n, q): degree and packed index of the coefficient; callers compute q from a moff[m] read before any store, since the
+// compiler cannot tell that the state stores leave the Geo tables unchanged and would read them again after each store
+__device__ inline void tend_store(const Geo &G, int stage, double dt, double nu4, int b, int n, size_t q, double2 o1, double2 o2, double2 o3,
                                   double2 o4, const TendIn &t, double phibar, double2 *__restrict__ x0, double2 *__restrict__ xs,
                                   double2 *__restrict__ acc, int mixed) {
-  const int n = m + k;
-  const size_t q = static_cast<size_t>(G.moff[m]) + k;
   double2 kz = o1, kd = cadd(o2, cscale(o3, n * (n + 1.0) / (G.a * G.a))), kf = o4;
   if (mixed) kf = csub(kf, cscale(t.d, phibar));
   if (q == 0) kz = kd = kf = cx(0, 0);  // sphere.cpp:387-389
@@ -76,11 +76,9 @@ __device__ inline void tend_store(const Geo &G, int stage, double dt, double nu4
 
 // mixed != 0: the phi-flux operands were formed with phi - phibar (phibar = phi_00 / sqrt 2, the area mean), and the
 // mean part -phibar D(U, V) = -phibar delta is added here from the stage input delta, exactly.
-__device__ inline void anal_epilogue(const Geo &G, int mode, int stage, double dt, double nu4, int b, int bl, int m, int k, double2 o1, double2 o2,
-                                     double2 o3, double2 o4, double2 *__restrict__ x0, double2 *__restrict__ xs, double2 *__restrict__ acc,
-                                     double2 *__restrict__ raw, int mixed) {
-  const int n = m + k;
-  const size_t q = static_cast<size_t>(G.moff[m]) + k;
+__device__ inline void anal_epilogue(const Geo &G, int mode, int stage, double dt, double nu4, int b, int bl, int n, size_t q, double2 o1,
+                                     double2 o2, double2 o3, double2 o4, double2 *__restrict__ x0, double2 *__restrict__ xs,
+                                     double2 *__restrict__ acc, double2 *__restrict__ raw, int mixed) {
   const size_t S = G.S;
   if (mode == ANAL_RAW) {  // raw outputs are indexed by the local member
     raw[(static_cast<size_t>(bl) * 4 + 0) * S + q] = o1;
@@ -101,7 +99,7 @@ __device__ inline void anal_epilogue(const Geo &G, int mode, int stage, double d
   }
   if (mode == ANAL_TEND) {
     const double phibar = mixed ? xs[G.at(b, 2, 0)].x * 0.70710678118654752440 : 0.0;  // phi_00 P_0^0
-    tend_store(G, stage, dt, nu4, b, m, k, o1, o2, o3, o4, tend_load(G, stage, b, q, x0, xs, acc, mixed), phibar, x0, xs, acc, mixed);
+    tend_store(G, stage, dt, nu4, b, n, q, o1, o2, o3, o4, tend_load(G, stage, b, q, x0, xs, acc, mixed), phibar, x0, xs, acc, mixed);
     return;
   }
   // ANAL_TENDOUT: the tendencies only, indexed by the local member
