@@ -1330,14 +1330,13 @@ void SphereCore::launch_stage(int stage, double dt, int b0, int nb) {
   }
 }
 
-// Members are independent, so an FP64 group of several 16-member tiles is stepped as up to kMaxParts parts of whole
-// tiles, each part's four stages a branch of its own: one part's kernels fill the gaps (tails, synchronisation
-// phases) of another's (32 x T170: 40.3 -> 45.1 member-days/s).  The mixed kernels gained nothing from it (112.3
-// against 112.2), so mixed groups, single members and tracers keep one branch, and the graph runs the same launches
-// as the timed eager steps.
+// Members are independent, so a group of several 16-member tiles is stepped as up to kMaxParts parts of whole tiles,
+// each part's four stages a branch of its own: one part's kernels fill the gaps (tails, synchronisation phases) of
+// another's (32 x T170: FP64 43.6 -> 51.3, BF16x6 124.5 -> 130.0 member-days/s).  Single members and tracers keep
+// one branch.
 int SphereCore::step_parts() const {
   const int tiles = (B + 15) / 16;
-  return has_tracer || advect_only || precision != LMW_FP64 ? 1 : std::min(tiles, kMaxParts);
+  return has_tracer || advect_only ? 1 : std::min(tiles, kMaxParts);
 }
 
 void SphereCore::enqueue_step(double dt) {
