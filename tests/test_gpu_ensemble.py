@@ -60,6 +60,19 @@ def test_persistent_macro_steps_equal_branch_macro_steps(lmw, precision):
         a.macro_steps([600.0, 200.0, 150.0], [1, 3, 4], 1)
 
 
+def test_persistent_many_tiles_equal_branch_macro_steps(lmw):
+    """Groups of several 16-member tiles, the last one partial (20 x T42, 17 x T85, 18 x T170): the blocks of the
+    persistent kernels walk lists of several items, the T170 inverse transform's items cover half the latitudes, and
+    the branch mode steps each group as concurrent member parts; every member bit-identical."""
+    truncs = [42] * 20 + [85] * 17 + [170] * 18
+    a, b = build(lmw, truncs, 3), build(lmw, truncs, 3)
+    a.set_persistent(True)
+    a.macro_steps([600.0, 300.0, 150.0], [1, 2, 4], 2)
+    b.macro_steps([600.0, 300.0, 150.0], [1, 2, 4], 2)
+    for x, y in zip(states(a), states(b)):
+        assert np.array_equal(x, y)
+
+
 def test_every_tile_variant_matches_default(lmw):
     """Mixed BF16x3: each (row chunk) x (latitude chunk, waves) variant stays within the BF16x3 error of the default
     shapes after 4 steps; FP64: the members-per-block variants are bit-identical (members are independent)."""
