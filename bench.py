@@ -93,9 +93,9 @@ def kernel_costs(J, K, S, M1, members, mixed, orography_planes=1):
 
 
 def timed_batch(ens, steps, warmup):
-    """Wall time of `steps` RK4 steps on the production path (one HIP graph replay per step; synchronous), then the
-    per-kernel times of as many steps again with HIP events around every kernel on the library's stream (the events
-    add a few microseconds between kernels, so they stay out of the timed region)."""
+    """Wall time of `steps` RK4 steps on the production path (replays of a HIP graph of 8 steps, then of a one-step graph
+    for the rest; synchronous), then the per-kernel times of as many steps again with HIP events around every kernel on
+    the library's stream (the events add a few microseconds between kernels, so they stay out of the timed region)."""
     ens.steps([DT], [warmup])
     t0 = time.perf_counter()
     ens.steps([DT], [steps])

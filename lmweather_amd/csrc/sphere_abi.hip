@@ -461,10 +461,7 @@ lmw_status lmw_sphere_set_diffusion(lmw_sphere *s, double nu4) {
   return guard([&] {
     NEED(s);
     s->core->prm.diffusion = nu4;
-    if (s->core->graph) {
-      (void)hipGraphExecDestroy(s->core->graph);
-      s->core->graph = nullptr;
-    }
+    s->core->drop_graph();
   });
 }
 lmw_status lmw_sphere_mark_reference(lmw_sphere *s) {
@@ -933,7 +930,7 @@ static void capture_macro(lmw_ensemble *e, const double *dts, const int *substep
   for (size_t i = 0; i < ng; i++) {
     SphereCore &c = *e->groups[i];
     LMW_HIP(hipStreamWaitEvent(c.stream, e->events[0], 0));
-    for (int k = 0; k < substeps[i]; k++) c.enqueue_step(dts[i]);
+    c.enqueue_steps(dts[i], substeps[i]);  // the group's member parts forked once per macro step
     LMW_HIP(hipEventRecord(e->events[i + 1], c.stream));
     LMW_HIP(hipStreamWaitEvent(e->origin, e->events[i + 1], 0));
   }

@@ -108,7 +108,9 @@ struct SphereCore {
   bool has_tracer = false, advect_only = false;
   DevArray<double2> q0, qs, qacc, tq;
   DevArray<double> grid2, ref_q;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;       // one step
+  hipGraphExec_t graph_many = nullptr;  // kGraphSteps steps
+  static constexpr int kGraphSteps = 8;
   double graph_dt = 0;
   // tile variant indices (defaults: FP64 4 members per block, RC 32, KC 64 with 4 waves)
   int tile_fp64_synth = 1, tile_fp64_anal = 1, tile_synth = 0, tile_anal = 1;
@@ -161,6 +163,7 @@ struct SphereCore {
   // stepping (all members)
   void launch_stage(int stage, double dt, int b0, int nb);
   void enqueue_step(double dt);
+  void enqueue_steps(double dt, int nsteps);  // member parts forked once for all the steps
   void steps(double dt, int64_t count);
   void steps_graph(double dt, int64_t count);
   // eager steps with a HIP event pair around every launch; ms[0..2] += total milliseconds of the inverse Legendre,

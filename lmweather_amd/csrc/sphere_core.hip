@@ -966,6 +966,7 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
 
 SphereCore::~SphereCore() {
   if (graph) (void)hipGraphExecDestroy(graph);
+  if (graph_many) (void)hipGraphExecDestroy(graph_many);
   for (int p = 1; p < kMaxParts; p++) {
     if (part_stream[p]) (void)hipStreamDestroy(part_stream[p]);
     if (part_join[p]) (void)hipEventDestroy(part_join[p]);
@@ -1194,10 +1195,7 @@ void SphereCore::set_precision(int p) {
     sync();
   }
   precision = p;
-  if (graph) {
-    LMW_HIP(hipGraphExecDestroy(graph));
-    graph = nullptr;
-  }
+  drop_graph();
 }
 
 void SphereCore::set_state_grid(int b, const double *h, const double *u, const double *v) {
@@ -1339,7 +1337,9 @@ int SphereCore::step_parts() const {
   return has_tracer || advect_only ? 1 : std::min(tiles, kMaxParts);
 }
 
-void SphereCore::enqueue_step(double dt) {
+void SphereCore::enqueue_step(double dt) { enqueue_steps(dt, 1); }
+
+void SphereCore::enqueue_steps(double dt, int nsteps) {
   const int parts = step_parts();
   if (parts > 1) {
     const int tiles = (B + 15) / 16;
@@ -1358,7 +1358,8 @@ void SphereCore::enqueue_step(double dt) {
         LMW_HIP(hipStreamWaitEvent(part_stream[p], part_fork, 0));
         stream = part_stream[p];  // the launch functions enqueue on `stream`
       }
-      for (int stage = 1; stage <= 4; stage++) launch_stage(stage, dt, b0, b1 - b0);
+      for (int i = 0; i < nsteps; i++)
+        for (int stage = 1; stage <= 4; stage++) launch_stage(stage, dt, b0, b1 - b0);
       if (p > 0) {
         LMW_HIP(hipEventRecord(part_join[p], stream));
         stream = main;
@@ -1367,20 +1368,23 @@ void SphereCore::enqueue_step(double dt) {
     for (int p = 1; p < parts; p++) LMW_HIP(hipStreamWaitEvent(stream, part_join[p], 0));
     return;
   }
-  // the tracer tendency reads the stage state before the dynamics' stage kernel advances it (sphere.cpp:412-427)
-  for (int stage = 1; stage <= 4; stage++) {
-    if (has_tracer) launch_tracer_tendency();
-    if (!advect_only) launch_stage(stage, dt, 0, B);
-    if (has_tracer) launch_tracer_update(stage, dt);
+  for (int i = 0; i < nsteps; i++) {
+    // the tracer tendency reads the stage state before the dynamics' stage kernel advances it (sphere.cpp:412-427)
+    for (int stage = 1; stage <= 4; stage++) {
+      if (has_tracer) launch_tracer_tendency();
+      if (!advect_only) launch_stage(stage, dt, 0, B);
+      if (has_tracer) launch_tracer_update(stage, dt);
+    }
+    if (advect_only && prm.diffusion > 0) launch_state_filter(dt);
   }
-  if (advect_only && prm.diffusion > 0) launch_state_filter(dt);
 }
 
 void SphereCore::drop_graph() {
-  if (graph) {
-    LMW_HIP(hipGraphExecDestroy(graph));
-    graph = nullptr;
-  }
+  for (hipGraphExec_t *ge : {&graph, &graph_many})
+    if (*ge) {
+      LMW_HIP(hipGraphExecDestroy(*ge));
+      *ge = nullptr;
+    }
 }
 
 void SphereCore::launch_tracer_tendency() {
@@ -1482,25 +1486,32 @@ double SphereCore::tracer_abs_sum() {
 }
 
 void SphereCore::steps(double dt, int64_t count) {
-  for (int64_t i = 0; i < count; i++) enqueue_step(dt);
+  for (int64_t i = 0; i < count; i += kGraphSteps) enqueue_steps(dt, static_cast<int>(std::min<int64_t>(kGraphSteps, count - i)));
 }
 
+// Replays of a graph of kGraphSteps steps, then of the one-step graph for the rest.  A graph of several steps forks
+// the member parts once, so each part runs its steps without waiting for the others at every step, and the device
+// crosses fewer graph boundaries (32 x T170 BF16x6: 130.2 member-days/s with one step per graph, 136.2 with 4, 136.9
+// with 8).
 void SphereCore::steps_graph(double dt, int64_t count) {
   if (count <= 0) return;
-  if (!graph || graph_dt != dt) {
-    if (graph) {
-      LMW_HIP(hipGraphExecDestroy(graph));
-      graph = nullptr;
-    }
+  auto capture = [&](hipGraphExec_t &ge, int nsteps) {
     hipGraph_t g;
     LMW_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(dt);
+    enqueue_steps(dt, nsteps);
     LMW_HIP(hipStreamEndCapture(stream, &g));
-    LMW_HIP(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+    LMW_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     LMW_HIP(hipGraphDestroy(g));
-    graph_dt = dt;
+  };
+  if (graph_dt != dt) drop_graph();
+  graph_dt = dt;
+  int64_t i = 0;
+  if (count >= kGraphSteps) {
+    if (!graph_many) capture(graph_many, kGraphSteps);
+    for (; i + kGraphSteps <= count; i += kGraphSteps) LMW_HIP(hipGraphLaunch(graph_many, stream));
   }
-  for (int64_t i = 0; i < count; i++) LMW_HIP(hipGraphLaunch(graph, stream));
+  if (i < count && !graph) capture(graph, 1);
+  for (; i < count; i++) LMW_HIP(hipGraphLaunch(graph, stream));
 }
 
 void SphereCore::steps_timed(double dt, int64_t count, double ms[3]) {
