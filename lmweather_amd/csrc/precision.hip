@@ -53,6 +53,19 @@ constexpr int nsplit(int prec) { return prec == LMW_BF16X6 ? 3 : 2; }
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Waits until this wave's vector memory loads are done, loads into LDS included (s_waitcnt vmcnt(0))
+__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Block barrier for LDS data only: this wave's LDS accesses complete, then s_barrier; the "memory" clobbers keep
+// the compiler from moving memory accesses across it.  Unlike __syncthreads() it does not wait for the wave's
+// outstanding global loads, among them loads into LDS (global_load_lds) whose buffer only the issuing wave reads.
+__device__ inline void lds_barrier() {
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+}
 // split_bf16x3 of two values at once, each part of the pair packed into one dword (a in the low half): one packed
 // conversion per part for both values, and the part's FP32 value recovered from the packed bits by a shift or a mask
 // (the same round-to-nearest-even parts as split_bf16x3; 11 instructions per pair instead of 15)
@@ -406,8 +419,9 @@ template <int PREC> using AnalIn = typename std::conditional<PREC == LMW_FP64, d
 // owning row tiles wave, wave + NW, ... (TPW of them: 8 tiles of 16 rows cover the largest parity block, T170).
 // The work of one block (order m, parity p, members mb .. mb + 15), shared by the per-group and multi-group kernels.
 // PREC = LMW_FP64 runs the same schedule on FP64 operands with the FP64 MFMA (16x16x4): the FP64 step's forward
-// transform on matrix cores.
-template <int PREC, int KC, int NW>
+// transform on matrix cores.  REC_LDS: the split-BF16 records through the LDS buffer described below (off in the
+// multi-group kernels, whose register budget it would overrun).
+template <int PREC, int KC, int NW, bool REC_LDS = true>
 __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, const AnalIn<PREC> *__restrict__ Gb, int mode, int stage,
                                               double dt, double nu4, double2 *__restrict__ x0, double2 *__restrict__ xs,
                                               double2 *__restrict__ acc, double2 *__restrict__ raw, int b0, int nb, int mixed, int m, int p,
@@ -426,6 +440,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   if (Rp == 0) return;
   const int nrt = Rp / 16, KK = 2 * G.Jhp;
   const size_t moff_m = static_cast<size_t>(G.moff[m]);  // read once, before the epilogue's stores
+  const int poff_m = G.poff[m];                          // and before the K loop's barriers
   AV accv[TPW][8];
   for (int t = 0; t < TPW; t++)
     for (int q = 0; q < 8; q++) accv[t][q] = AV{0, 0, 0, 0};
@@ -436,19 +451,56 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   ST vv[NIT][2][8];
   float4 rr[NIT][5];  // mixed: the raw records of the next chunk
   bool rr_ok[NIT];
+  // mixed: the thread of latitude pair kk / 2 reads that pair's one 80-byte record (five weighted coefficients per
+  // latitude) from a clamped, valid address whatever its lane (rec_ok marks the lanes whose record is used), so that
+  // every chunk issues the same loads in the same order and the waits for them stay exact
+  auto rec_ok = [&](int it, int c0) {
+    const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM);
+    return e < (KC / 2) * MEM && mb + j < nb && ((c0 + kk) >> 1) < G.Jh;
+  };
+  auto rec_src = [&](int it, int c0) {
+    const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM);
+    const int blc = min(mb + j, nb - 1), jhc = min((c0 + kk) >> 1, G.Jh - 1);
+    return reinterpret_cast<const float4 *>(Gb + ((static_cast<size_t>(b0 + blc) * G.M1 + m) * G.Jh + jhc) * 10);
+  };
+  // Split-BF16 modes with 32-latitude chunks and 4 waves (one record per thread): the chunk's records are requested
+  // as contiguous 1 KB pieces, one per instruction, straight into an LDS buffer (buffer_load_dwordx4 ... lds: lane
+  // i's 16 bytes land at base + 16 i), and each thread then reads its own record from the buffer.  The chunk's
+  // records of one member are 16 consecutive 80-byte records (1280 bytes), held at bytes 1280 j .. 1280 j + 1279.
+  // Loaded one record per thread instead, a request of 16 bytes per lane touched 16 members' records at once, about
+  // 48 cache lines per instruction (T170 x 32: 41.7 -> 36.7 us per launch).  Threads take (member e / 16, pair
+  // e % 16) here.  The next chunk's records are requested once every wave has read this chunk's (after the staging
+  // barrier), so they arrive during the MFMAs.  The buffer is an LDS object of its own, apart from the operand
+  // chunk, so the waits the compiler counts for these loads stay on the buffer's reads; the raw-buffer form (not the
+  // global one) is what the compiler counts exactly.  The resource starts at the block's first record (member
+  // b0 + mb, order m): every clamped record lies at or after it, within 32-bit offsets.  The FP32 mode keeps the
+  // register loads: its table loads in the MFMA phase would wait for these requests (77 -> 87 us).
+  constexpr bool DMA = REC_LDS && (PREC == LMW_BF16X6 || PREC == LMW_BF16X3) && KC == 32 && NW == 4;
+  static_assert(!DMA || (KC / 2 == 16 && MEM == 16 && NIT == 1), "one record per thread, 16 pairs x 16 members");
+  __shared__ float4 recbuf[DMA ? NW * 5 * 64 : 1];
+  auto block_sync = [&]() {
+    if constexpr (DMA) lds_barrier();
+    else __syncthreads();
+  };
+  const __amdgpu_buffer_rsrc_t rec_rs =  // (unused without DMA)
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<GT *>(Gb + (static_cast<size_t>(b0 + mb) * G.M1 + m) * G.Jh * 10), 0, 0x7fffffff, 0x00020000);
+  auto dma_issue = [&](int c0) {
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      const int i = (wave * 5 + c) * 1024 + lane * 16;  // byte of the chunk: member i / 1280, pair (i % 1280) / 80
+      const int j = i / 1280, o = i - j * 1280, r = o / 80;
+      const int jl = min(j, nb - 1 - mb), jhc = min((c0 >> 1) + r, G.Jh - 1);  // (clamped pieces are never used)
+      const unsigned off = (static_cast<unsigned>(jl) * G.M1 * G.Jh + jhc) * 80u + (o - r * 80);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rec_rs, (lds_void *)(recbuf + (wave * 5 + c) * 64), 16, off, 0, 0, 0);
+    }
+  };
   auto stage_load = [&](int c0) {
     if constexpr (PREC != LMW_FP64) {
-      // mixed: the thread of latitude pair kk / 2 loads that pair's one 80-byte record (five weighted coefficients
-      // per latitude) and keeps it raw until the chunk is written to LDS, so that nothing waits for the load here
-      // The loads are issued unconditionally (from a clamped, valid record; rr_ok marks the lanes whose record is
-      // used), so that every chunk issues the same loads in the same order and the waits for them stay exact.
+      // registers: the raw records are kept until the chunk is written to LDS, so that nothing waits for them here
 #pragma unroll
       for (int it = 0; it < NIT; it++) {
-        const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
-        const int jh = (c0 + kk) >> 1;
-        rr_ok[it] = e < (KC / 2) * MEM && bl < nb && jh < G.Jh;
-        const int blc = min(bl, nb - 1), jhc = min(jh, G.Jh - 1);
-        const float4 *src = reinterpret_cast<const float4 *>(Gb + ((static_cast<size_t>(b0 + blc) * G.M1 + m) * G.Jh + jhc) * 10);
+        rr_ok[it] = rec_ok(it, c0);
+        const float4 *src = rec_src(it, c0);
 #pragma unroll
         for (int c = 0; c < 5; c++) rr[it][c] = src[c];
       }
@@ -477,15 +529,17 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   // then wait for the tables only)
   constexpr bool PIPE = (PREC == LMW_BF16X6 || PREC == LMW_BF16X3) && KC == 32;
   v4f araw[TPW][2];
-  stage_load(0);
+  if constexpr (DMA) dma_issue(0);
+  else stage_load(0);
   for (int c0 = 0; c0 < KK; c0 += KC) {
-    __syncthreads();
+    if constexpr (DMA) wait_vmem();  // this wave's pieces of the chunk's records (the only loads outstanding here)
+    block_sync();                    // (and, for DMA, everyone's pieces)
     if constexpr (PIPE) {
 #pragma unroll
       for (int ti = 0; ti < TPW; ti++) {
         const int t = min(wave + NW * ti, nrt - 1);  // (a wave without this tile loads a valid row it does not use)
         {
-          const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + rbase + 16 * t + (lane & 15)) + c0 + 8 * (lane >> 4);
+          const size_t ai = static_cast<size_t>(KK) * (poff_m + rbase + 16 * t + (lane & 15)) + c0 + 8 * (lane >> 4);
           if constexpr (PREC == LMW_BF16X6) {
             araw[ti][0] = *reinterpret_cast<const v4f *>(T.Abf + ai);
             araw[ti][1] = *reinterpret_cast<const v4f *>(T.Abf + ai + 4);
@@ -496,9 +550,17 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
         }
       }
     }
+    if constexpr (DMA) {
+      // this chunk's records from the buffer (the table requests above stay in flight)
+      const int jd = threadIdx.x >> 4, jp = threadIdx.x & 15;
+      rr_ok[0] = mb + jd < nb && (c0 >> 1) + jp < G.Jh;
+#pragma unroll
+      for (int c = 0; c < 5; c++) rr[0][c] = recbuf[jd * 80 + jp * 5 + c];
+    }
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
-      const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM);
+      const int e = threadIdx.x + 64 * NW * it;
+      const int j = DMA ? e >> 4 : e % MEM, kk = DMA ? 2 * (e & 15) : 2 * (e / MEM);  // (member, first k of the pair)
       if (e >= (KC / 2) * MEM) continue;
       if constexpr (PREC != LMW_FP64) {
         // mixed: K runs over (latitude pair, P | H) interleaved, so the record of pair kk / 2 gives both its P
@@ -543,8 +605,10 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
         }
       }
     }
-    stage_load(c0 + KC < KK ? c0 + KC : c0);  // (the last chunk requests itself again: the same load count)
-    __syncthreads();
+    if constexpr (!DMA) stage_load(c0 + KC < KK ? c0 + KC : c0);  // (the last chunk requests itself again: the same load count)
+    block_sync();
+    // every wave has read its records: the next chunk's go to the buffer during this chunk's MFMAs
+    if constexpr (DMA) dma_issue(c0 + KC < KK ? c0 + KC : c0);  // (the last chunk requests itself again: the same load count)
 #pragma unroll
     for (int ti = 0; ti < TPW; ti++) {
       const int t = wave + NW * ti;
@@ -560,7 +624,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
           double av[FS];
 #pragma unroll
           for (int i = 0; i < FS; i++)
-            av[i] = T.A64[static_cast<size_t>(KK) * G.poff[m] + static_cast<size_t>(c0 + 4 * (s0 + i) + (lane >> 4)) * kp + r];
+            av[i] = T.A64[static_cast<size_t>(KK) * poff_m + static_cast<size_t>(c0 + 4 * (s0 + i) + (lane >> 4)) * kp + r];
 #pragma unroll
           for (int i = 0; i < FS; i++) {
             const int kk = 4 * (s0 + i) + (lane >> 4);
@@ -572,13 +636,13 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       } else if constexpr (PREC == LMW_FP32) {
         for (int s = 0; s < kc / 4; s++) {
           const int kk = 4 * s + (lane >> 4);
-          const float av = T.A32[static_cast<size_t>(KK) * G.poff[m] + static_cast<size_t>(c0 + kk) * kp + r];
+          const float av = T.A32[static_cast<size_t>(KK) * poff_m + static_cast<size_t>(c0 + kk) * kp + r];
 #pragma unroll
           for (int q = 0; q < 8; q++) accv[ti][q] = mma32(av, bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
         }
       } else {
         for (int s = 0; s < kc / 32; s++) {
-          const size_t ai = static_cast<size_t>(KK) * (G.poff[m] + r) + c0 + 32 * s + 8 * (lane >> 4);
+          const size_t ai = static_cast<size_t>(KK) * (poff_m + r) + c0 + 32 * s + 8 * (lane >> 4);
           if constexpr (PREC == LMW_BF16X6) {
             v8bf ah, al, a3;
             if constexpr (PIPE) {
@@ -610,6 +674,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       }
     }
   }
+  if constexpr (DMA) wait_vmem();  // the last chunk's repeated request into the LDS buffer
   const int j = lane & 15, bl = mb + j;
   if (bl >= nb) return;
   const int b = b0 + bl, Km = G.N - m + 1;
@@ -839,7 +904,7 @@ __global__ __launch_bounds__(512, NW == 4 ? 3 : 1) void k_anal_mx_multi(MultiAna
   for (int w = M.start[blockIdx.x]; w < M.start[blockIdx.x + 1]; w++) {
     const int4 it = M.work[w];
     const AnalGroup &g = M.g[it.x];
-    anal_mx_block<PREC, KC, NW>(g.G, g.T, g.Gb, ANAL_TEND, M.stage, g.dt, g.nu4, g.x0, g.xs, g.acc, nullptr, 0, g.nb, 1, it.y, it.z, it.w);
+    anal_mx_block<PREC, KC, NW, false>(g.G, g.T, g.Gb, ANAL_TEND, M.stage, g.dt, g.nu4, g.x0, g.xs, g.acc, nullptr, 0, g.nb, 1, it.y, it.z, it.w);
   }
 }
 
