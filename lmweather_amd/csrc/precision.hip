@@ -55,6 +55,9 @@ typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
 
+// Member column swizzle of the FP64 forward transform's operand chunk (row q, k): distinct for the 16 (q / 2, k / 2 mod 4)
+__device__ inline int f64_swz(int q, int k) { return (((q >> 1) << 2) + ((k >> 1) & 3)) & 15; }
+
 // Waits until this wave's vector memory loads are done, loads into LDS included (s_waitcnt vmcnt(0))
 __device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
@@ -449,6 +452,9 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   // registers while the current chunk's MFMAs run.
   constexpr int NIT = ((KC / 2) * MEM + 64 * NW - 1) / (64 * NW);
   ST vv[NIT][2][8];
+  constexpr int NP64 = PREC == LMW_FP64 ? (MEM * KC * 4 + 64 * NW - 1) / (64 * NW) : 1;
+  GT pv[NP64];  // FP64: the next chunk's operand pieces
+  bool pok[NP64];
   float4 rr[NIT][5];  // mixed: the raw records of the next chunk
   bool rr_ok[NIT];
   // mixed: the thread of latitude pair kk / 2 reads that pair's one 80-byte record (five weighted coefficients per
@@ -506,22 +512,17 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       }
       return;
     }
+    // FP64: the chunk's operands are MEM members x KC k x 4 complex values, each member's KC x 4 contiguous within a
+    // part (64 bytes per latitude); request slot i of a thread takes piece P = (i NW + wave) 64 + lane, so that each
+    // request of a wave reads 1 KB of one member's operands (one complex value per lane at a time touched 64 cache
+    // lines per request).  Loads from clamped valid addresses; pok marks the pieces that are used.
 #pragma unroll
-    for (int it = 0; it < NIT; it++) {
-      const int e = threadIdx.x + 64 * NW * it, j = e % MEM, kk = 2 * (e / MEM), bl = mb + j;
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int k = c0 + kk + h;
-        const int part = k >= G.Jhp ? 1 : 0, jh = part ? k - G.Jhp : k;
-        GT g[4] = {GT{0, 0}, GT{0, 0}, GT{0, 0}, GT{0, 0}};
-        if (e < (KC / 2) * MEM && bl < nb && jh < G.Jh) {
-          const GT *src = Gb + (((static_cast<size_t>(b0 + bl) * G.M1 + m) * 2 + p) * 2 + part) * 4 * G.Jh + static_cast<size_t>(jh) * 4;
-#pragma unroll
-          for (int c = 0; c < 4; c++) g[c] = src[c];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) vv[it][h][q] = (q & 1) ? g[q >> 1].y : g[q >> 1].x;
-      }
+    for (int i = 0; i < NP64; i++) {
+      const int P = (i * NW + wave) * 64 + lane, j = P / (4 * KC), kl = (P >> 2) % KC, c = P & 3;
+      const int k = c0 + kl, part = k >= G.Jhp ? 1 : 0, jh = part ? k - G.Jhp : k;
+      pok[i] = P < MEM * KC * 4 && mb + j < nb && jh < G.Jh;
+      const int jl = max(0, min(j, nb - 1 - mb)), jhc = min(jh, G.Jh - 1);
+      pv[i] = Gb[(((static_cast<size_t>(b0 + mb + jl) * G.M1 + m) * 2 + p) * 2 + part) * 4 * G.Jh + static_cast<size_t>(jhc) * 4 + c];
     }
   };
   // split BF16 with 32-wide chunks: this chunk's table operands (FP32 entries for BF16X6, their two BF16 parts for
@@ -557,8 +558,22 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
 #pragma unroll
       for (int c = 0; c < 5; c++) rr[0][c] = recbuf[jd * 80 + jp * 5 + c];
     }
+    if constexpr (PREC == LMW_FP64) {
+      // operand (member j, k, value c) -> rows q = 2c (re), 2c + 1 (im) of the chunk, member column j ^ f64_swz(q, k):
+      // a wave's 64 pieces then go to distinct LDS banks, and the MFMA's reads of a row stay one full row
+#pragma unroll
+      for (int i = 0; i < NP64; i++) {
+        const int P = (i * NW + wave) * 64 + lane, j = P / (4 * KC), kl = (P >> 2) % KC, c = P & 3;
+        if (P >= MEM * KC * 4) continue;
+        const GT v = pok[i] ? pv[i] : GT{0, 0};
+        const int col = j ^ f64_swz(2 * c, kl);
+        bf[((2 * c) * KC + kl) * MEM + col] = v.x;
+        bf[((2 * c + 1) * KC + kl) * MEM + col] = v.y;
+      }
+    }
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
+      if constexpr (PREC == LMW_FP64) break;
       const int e = threadIdx.x + 64 * NW * it;
       const int j = DMA ? e >> 4 : e % MEM, kk = DMA ? 2 * (e & 15) : 2 * (e / MEM);  // (member, first k of the pair)
       if (e >= (KC / 2) * MEM) continue;
@@ -629,7 +644,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
           for (int i = 0; i < FS; i++) {
             const int kk = 4 * (s0 + i) + (lane >> 4);
 #pragma unroll
-            for (int q = 0; q < 8; q++) accv[ti][q] = mma64(av[i], bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
+            for (int q = 0; q < 8; q++) accv[ti][q] = mma64(av[i], bf[(q * KC + kk) * MEM + ((lane & 15) ^ f64_swz(q, kk))], accv[ti][q]);
           }
         }
         (void)kc;
