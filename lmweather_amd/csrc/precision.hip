@@ -643,8 +643,14 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
 #pragma unroll
           for (int i = 0; i < FS; i++) {
             const int kk = 4 * (s0 + i) + (lane >> 4);
+            // o3 (q = 4, 5: the Laplacian part of ddelta) has no H term, so its operands in the H half of K are zeros
+            // (Jhp is a multiple of 16: a k-step of 4 lies in one half)
+            const bool hpart = c0 + 4 * (s0 + i) >= G.Jhp;
 #pragma unroll
-            for (int q = 0; q < 8; q++) accv[ti][q] = mma64(av[i], bf[(q * KC + kk) * MEM + ((lane & 15) ^ f64_swz(q, kk))], accv[ti][q]);
+            for (int q = 0; q < 8; q++) {
+              if ((q == 4 || q == 5) && hpart) continue;
+              accv[ti][q] = mma64(av[i], bf[(q * KC + kk) * MEM + ((lane & 15) ^ f64_swz(q, kk))], accv[ti][q]);
+            }
           }
         }
         (void)kc;
