@@ -530,11 +530,21 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   // then wait for the tables only)
   constexpr bool PIPE = (PREC == LMW_BF16X6 || PREC == LMW_BF16X3) && KC == 32;
   v4f araw[TPW][2];
+  // FP64: the table operands are requested FS64 k-steps at a time in the MFMA phase (the first FS64 of the first
+  // tile before the staging, so that they have arrived when the MFMAs start)
+  constexpr int FS64 = TPW > 1 ? 4 : 8;  // (two tiles per wave leave registers for four)
+  double av0[FS64];
   if constexpr (DMA) dma_issue(0);
   else stage_load(0);
   for (int c0 = 0; c0 < KK; c0 += KC) {
     if constexpr (DMA) wait_vmem();  // this wave's pieces of the chunk's records (the only loads outstanding here)
     block_sync();                    // (and, for DMA, everyone's pieces)
+    if constexpr (PREC == LMW_FP64) {
+      const int r0 = rbase + 16 * min(wave, nrt - 1) + (lane & 15);  // (a wave without a tile loads a valid row)
+#pragma unroll
+      for (int i = 0; i < FS64; i++)
+        av0[i] = T.A64[static_cast<size_t>(KK) * poff_m + static_cast<size_t>(c0 + 4 * i + (lane >> 4)) * kp + r0];
+    }
     if constexpr (PIPE) {
 #pragma unroll
       for (int ti = 0; ti < TPW; ti++) {
@@ -633,15 +643,9 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       if constexpr (PREC == LMW_FP64) {
         // the tile's table operands of this chunk are requested FS k-steps at a time, so the MFMAs wait for one
         // round trip per FS k-steps instead of one per k-step (kc == KC: KK is a multiple of 32)
-        constexpr int FS = TPW > 1 ? 4 : 8;  // (two tiles per wave leave registers for four)
-#pragma unroll 1
-        for (int s0 = 0; s0 < KC / 4; s0 += FS) {
-          double av[FS];
+        auto mma_steps = [&](const double (&av)[FS64], int s0) {
 #pragma unroll
-          for (int i = 0; i < FS; i++)
-            av[i] = T.A64[static_cast<size_t>(KK) * poff_m + static_cast<size_t>(c0 + 4 * (s0 + i) + (lane >> 4)) * kp + r];
-#pragma unroll
-          for (int i = 0; i < FS; i++) {
+          for (int i = 0; i < FS64; i++) {
             const int kk = 4 * (s0 + i) + (lane >> 4);
             // o3 (q = 4, 5: the Laplacian part of ddelta) has no H term, so its operands in the H half of K are zeros
             // (Jhp is a multiple of 16: a k-step of 4 lies in one half)
@@ -652,6 +656,19 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
               accv[ti][q] = mma64(av[i], bf[(q * KC + kk) * MEM + ((lane & 15) ^ f64_swz(q, kk))], accv[ti][q]);
             }
           }
+        };
+        int sb = 0;
+        if (ti == 0) {  // the first k-steps of the first tile: requested before the staging
+          mma_steps(av0, 0);
+          sb = FS64;
+        }
+#pragma unroll 1
+        for (int s0 = sb; s0 < KC / 4; s0 += FS64) {
+          double av[FS64];
+#pragma unroll
+          for (int i = 0; i < FS64; i++)
+            av[i] = T.A64[static_cast<size_t>(KK) * poff_m + static_cast<size_t>(c0 + 4 * (s0 + i) + (lane >> 4)) * kp + r];
+          mma_steps(av, s0);
         }
         (void)kc;
       } else if constexpr (PREC == LMW_FP32) {
