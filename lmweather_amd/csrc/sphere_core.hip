@@ -347,22 +347,22 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   const size_t hplane = static_cast<size_t>(flags >> 2);  // this member's orography plane
   // Every global load of the prologue is issued before the first value is used, from a clamped valid address where
   // a lane has nothing to load (its value is zeroed afterwards), so the loads are in flight together and the thread
-  // waits once: orography and f of this thread's longitudes of both rows (skipped for the whole block when the
-  // member has no orography or an untilted axis), the four fields' coefficients at m' and L - m' of each inverse-FFT
-  // input, the twiddles.  Converted from FP64 only after the coefficients have arrived.
+  // waits once: orography and f of this thread's longitudes of both rows, the four fields' coefficients at m' and
+  // L - m' of each inverse-FFT input, the twiddles.  Orography and f are loaded whatever the member's flags and
+  // selected only where they are used (loaded inside a branch on the flags, they were converted inside it, which
+  // waited for them at once); a member without orography or with an untilted axis reads the first row of plane 0
+  // instead (every group holds at least one plane), which all its blocks share from the cache.
   const int kc = k < L ? k : L - 1;
-  double hraw[2][2] = {{0, 0}, {0, 0}}, fraw[2][2] = {{0, 0}, {0, 0}};
-  if (!flat) {
+  double hraw[2][2], fraw[2][2];
 #pragma unroll
-    for (int s = 0; s < 2; s++)
+  for (int s = 0; s < 2; s++) {
+    const size_t hrow = flat ? 0 : hplane * G.J + (s ? js : jn);
+    const size_t frow = untilted ? 0 : static_cast<size_t>(b) * G.J + (s ? js : jn);
 #pragma unroll
-      for (int e = 0; e < 2; e++) hraw[s][e] = hs[(hplane * G.J + (s ? js : jn)) * K + 2 * kc + e];
-  }
-  if (!untilted) {
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int e = 0; e < 2; e++) fraw[s][e] = fgrid[(static_cast<size_t>(b) * G.J + (s ? js : jn)) * K + 2 * kc + e];
+    for (int e = 0; e < 2; e++) {
+      hraw[s][e] = hs[hrow * K + 2 * kc + e];
+      fraw[s][e] = fgrid[frow * K + 2 * kc + e];
+    }
   }
   // inverse real FFT inputs (the arithmetic of irfft_entry): per (row s, m') the four fields' coefficients at m'
   // and L - m', 4 contiguous complex values each
