@@ -672,11 +672,18 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
         }
         (void)kc;
       } else if constexpr (PREC == LMW_FP32) {
-        for (int s = 0; s < kc / 4; s++) {
-          const int kk = 4 * s + (lane >> 4);
-          const float av = T.A32[static_cast<size_t>(KK) * poff_m + static_cast<size_t>(c0 + kk) * kp + r];
+        // table operands requested 8 k-steps at a time (kc is a multiple of 32), one round trip per 8 k-steps
+        for (int s0 = 0; s0 < kc / 4; s0 += 8) {
+          float av[8];
 #pragma unroll
-          for (int q = 0; q < 8; q++) accv[ti][q] = mma32(av, bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
+          for (int i = 0; i < 8; i++)
+            av[i] = T.A32[static_cast<size_t>(KK) * poff_m + static_cast<size_t>(c0 + 4 * (s0 + i) + (lane >> 4)) * kp + r];
+#pragma unroll
+          for (int i = 0; i < 8; i++) {
+            const int kk = 4 * (s0 + i) + (lane >> 4);
+#pragma unroll
+            for (int q = 0; q < 8; q++) accv[ti][q] = mma32(av[i], bf[(q * KC + kk) * MEM + (lane & 15)], accv[ti][q]);
+          }
         }
       } else {
         for (int s = 0; s < kc / 32; s++) {
