@@ -252,7 +252,7 @@ lmw_status lmw_ensemble_diag(lmw_ensemble *e, int member, int which, double *out
  * ascending order; lmw_ensemble_groups lists them) */
 lmw_status lmw_ensemble_groups(const lmw_ensemble *e, int *ngroups, int *truncations, int *members_per_group);
 lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t *counts);
-/* capture one RK4 step of every group into a HIP graph and replay it (hipGraph path); counts per group */
+/* replay captured HIP graphs per group (8 RK4 steps, then single steps for the rest; hipGraph path); counts per group */
 lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts);
 /* `count` macro steps: in each, group g advances substeps[g] steps of dts[g] (e.g. 600 s = 1 x 600 s at T42,
  * 2 x 300 s at T85, 4 x 150 s at T170).  One macro step is captured once as a HIP graph whose branches are the

@@ -184,7 +184,7 @@ class Sphere:
         L.call("lmw_sphere_step", self._h, dt)
 
     def steps(self, dt: float, count: int) -> None:
-        """`count` steps of dt with no periodic checks (one HIP graph replay per step)."""
+        """`count` steps of dt with no periodic checks (replays of a HIP graph of 8 steps, then of a one-step graph)."""
         L.call("lmw_sphere_steps", self._h, dt, count)
 
     def run(self, seconds: float, dt: float) -> None:
