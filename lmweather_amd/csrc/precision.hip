@@ -476,11 +476,14 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   // Loaded one record per thread instead, a request of 16 bytes per lane touched 16 members' records at once, about
   // 48 cache lines per instruction (T170 x 32: 41.7 -> 36.7 us per launch).  Threads take (member e / 16, pair
   // e % 16) here.  The next chunk's records are requested once every wave has read this chunk's (after the staging
-  // barrier), so they arrive during the MFMAs.  The buffer is an LDS object of its own, apart from the operand
-  // chunk, so the waits the compiler counts for these loads stay on the buffer's reads; the raw-buffer form (not the
-  // global one) is what the compiler counts exactly.  The resource starts at the block's first record (member
-  // b0 + mb, order m): every clamped record lies at or after it, within 32-bit offsets.  The FP32 mode keeps the
-  // register loads: its table loads in the MFMA phase would wait for these requests (77 -> 87 us).
+  // barrier), so they arrive during the MFMAs.  Each wave waits for its own requests (vmcnt(0)) before the next
+  // chunk's first barrier, since the compiler places no wait before reads of such a buffer, and the barriers wait
+  // for LDS only (__syncthreads() would also wait for the requests in flight).  The buffer is an LDS object of its
+  // own, apart from the operand chunk, and the requests use the raw-buffer form (not the global one), whose
+  // outstanding count the compiler tracks exactly, so the waits for the table operands do not wait for the records.
+  // The resource starts at the block's first record (member b0 + mb, order m): every clamped record lies at or
+  // after it, within 32-bit offsets.  The FP32 mode keeps the register loads: its table loads in the MFMA phase
+  // would wait for these requests (77 -> 87 us).
   constexpr bool DMA = REC_LDS && (PREC == LMW_BF16X6 || PREC == LMW_BF16X3) && KC == 32 && NW == 4;
   static_assert(!DMA || (KC / 2 == 16 && MEM == 16 && NIT == 1), "one record per thread, 16 pairs x 16 members");
   __shared__ float4 recbuf[DMA ? NW * 5 * 64 : 1];
