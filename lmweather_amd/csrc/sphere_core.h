@@ -62,7 +62,7 @@ struct SphereCore {
   hipStream_t stream = nullptr;
   // member parts of a large group stepped as concurrent branches of the step (parts of whole 16-member tiles): the
   // second and later parts run on these streams, forked from and joined to `stream` with these events
-  static constexpr int kMaxParts = 4;
+  static constexpr int kMaxParts = 2;  // (C3, 741 -> 746 member-days/s against 4 parts; 8 parts: 680)
   hipStream_t part_stream[kMaxParts] = {};
   hipEvent_t part_fork = nullptr, part_join[kMaxParts] = {};
   int step_parts() const;

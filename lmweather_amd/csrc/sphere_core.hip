@@ -1330,8 +1330,8 @@ void SphereCore::launch_stage(int stage, double dt, int b0, int nb) {
 
 // Members are independent, so a group of several 16-member tiles is stepped as up to kMaxParts parts of whole tiles,
 // each part's four stages a branch of its own: one part's kernels fill the gaps (tails, synchronisation phases) of
-// another's (32 x T170: FP64 43.6 -> 51.3, BF16x6 124.5 -> 130.0 member-days/s).  Single members and tracers keep
-// one branch.
+// another's (32 x T170: FP64 43.6 -> 51.3, BF16x6 124.5 -> 130.0 member-days/s).  Two parts beat four and eight for
+// the larger groups of the uneven ensemble.  Single members and tracers keep one branch.
 int SphereCore::step_parts() const {
   const int tiles = (B + 15) / 16;
   return has_tracer || advect_only ? 1 : std::min(tiles, kMaxParts);
