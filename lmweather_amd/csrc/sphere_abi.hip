@@ -347,7 +347,9 @@ struct lmw_ensemble {
   // macro step: every group advanced by its own sub-steps, captured as one HIP graph whose branches are the groups'
   // streams (forked from and joined to `origin`)
   hipStream_t origin = nullptr;
-  hipGraphExec_t macro = nullptr;
+  hipGraphExec_t macro = nullptr;       // one macro step
+  hipGraphExec_t macro_many = nullptr;  // kMacroGraph macro steps (branch mode: each group's steps in one chain)
+  static constexpr int kMacroGraph = 4;
   std::vector<double> macro_dts;
   std::vector<int> macro_sub;
   std::vector<hipEvent_t> events;  // fork, then one join per group
@@ -356,6 +358,7 @@ struct lmw_ensemble {
   std::map<unsigned, std::unique_ptr<MultiWork>> work;  // per set of active groups (bit mask)
   ~lmw_ensemble() {
     if (macro) (void)hipGraphExecDestroy(macro);
+    if (macro_many) (void)hipGraphExecDestroy(macro_many);
     for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
     if (origin) (void)hipStreamDestroy(origin);
   }
@@ -912,17 +915,21 @@ lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t 
 lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts) {
   return guard([&] { ensemble_steps(e, dts, counts, true); });
 }
-// one graph per macro step: group g advances substeps[g] steps of dts[g]; the groups' branches run concurrently
-static void capture_macro(lmw_ensemble *e, const double *dts, const int *substeps) {
+// A graph of nmacro macro steps: group g advances nmacro x substeps[g] steps of dts[g] as one chain on its stream
+// (its member parts forked once), the groups' chains concurrent branches.  The groups are independent models, so a
+// graph of several macro steps lets the small groups' next macro steps run beside the largest group's last steps
+// instead of waiting at every macro step's join; each group's state advances through the same steps in the same
+// order either way.
+static void capture_macro_graph(lmw_ensemble *e, const double *dts, const int *substeps, int nmacro, hipGraphExec_t &ge) {
   const size_t ng = e->groups.size();
   if (!e->origin) {
     LMW_HIP(hipStreamCreate(&e->origin));
     e->events.resize(ng + 1);
     for (auto &ev : e->events) LMW_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   }
-  if (e->macro) {
-    LMW_HIP(hipGraphExecDestroy(e->macro));
-    e->macro = nullptr;
+  if (ge) {
+    LMW_HIP(hipGraphExecDestroy(ge));
+    ge = nullptr;
   }
   hipGraph_t g;
   LMW_HIP(hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
@@ -930,15 +937,19 @@ static void capture_macro(lmw_ensemble *e, const double *dts, const int *substep
   for (size_t i = 0; i < ng; i++) {
     SphereCore &c = *e->groups[i];
     LMW_HIP(hipStreamWaitEvent(c.stream, e->events[0], 0));
-    c.enqueue_steps(dts[i], substeps[i]);  // the group's member parts forked once per macro step
+    c.enqueue_steps(dts[i], substeps[i] * nmacro);  // the group's member parts forked once for all its steps
     LMW_HIP(hipEventRecord(e->events[i + 1], c.stream));
     LMW_HIP(hipStreamWaitEvent(e->origin, e->events[i + 1], 0));
   }
   LMW_HIP(hipStreamEndCapture(e->origin, &g));
-  LMW_HIP(hipGraphInstantiate(&e->macro, g, nullptr, nullptr, 0));
+  LMW_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
   LMW_HIP(hipGraphDestroy(g));
-  e->macro_dts.assign(dts, dts + ng);
-  e->macro_sub.assign(substeps, substeps + ng);
+}
+static void capture_macro(lmw_ensemble *e, const double *dts, const int *substeps) {
+  drop_macro(e);
+  capture_macro_graph(e, dts, substeps, 1, e->macro);
+  e->macro_dts.assign(dts, dts + e->groups.size());
+  e->macro_sub.assign(substeps, substeps + e->groups.size());
 }
 // Persistent variant of the macro step: ticks of the finest group's step; at each tick the groups due a step run
 // their four stages together, each stage as one multi-group inverse-transform launch, the groups' grid-point
@@ -1027,7 +1038,12 @@ lmw_status lmw_ensemble_macro_steps(lmw_ensemble *e, const double *dts, const in
       if (e->persistent) capture_macro_persistent(e, dts, substeps);
       else capture_macro(e, dts, substeps);
     }
-    for (int64_t k = 0; k < count; k++) LMW_HIP(hipGraphLaunch(e->macro, e->origin));
+    int64_t k = 0;
+    if (!e->persistent && count >= lmw_ensemble::kMacroGraph) {  // graphs of kMacroGraph macro steps, then single ones
+      if (!e->macro_many) capture_macro_graph(e, dts, substeps, lmw_ensemble::kMacroGraph, e->macro_many);
+      for (; k + lmw_ensemble::kMacroGraph <= count; k += lmw_ensemble::kMacroGraph) LMW_HIP(hipGraphLaunch(e->macro_many, e->origin));
+    }
+    for (; k < count; k++) LMW_HIP(hipGraphLaunch(e->macro, e->origin));
     LMW_HIP(hipStreamSynchronize(e->origin));
     for (size_t i = 0; i < ng; i++) {
       SphereCore &c = *e->groups[i];
@@ -1070,6 +1086,10 @@ void drop_macro(lmw_ensemble *e) {
   if (e && e->macro) {
     LMW_HIP(hipGraphExecDestroy(e->macro));
     e->macro = nullptr;
+  }
+  if (e && e->macro_many) {
+    LMW_HIP(hipGraphExecDestroy(e->macro_many));
+    e->macro_many = nullptr;
   }
 }
 SphereCore &single_group(lmw_ensemble *e) {

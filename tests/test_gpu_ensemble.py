@@ -34,15 +34,16 @@ def states(e):
     return [np.stack([e.get_spectral(i, f) for f in range(3)]) for i in range(len(e.truncations))]
 
 
-@pytest.mark.parametrize("precision", [0, 2])
-def test_macro_steps_equal_group_steps(lmw, precision):
+@pytest.mark.parametrize("precision,count", [(0, 3), (2, 3), (0, 9), (3, 9)])
+def test_macro_steps_equal_group_steps(lmw, precision, count):
+    """count 9: two replays of the graph of 4 macro steps, then one of the single macro step."""
     truncs = [42] * 5 + [85] * 3 + [170] * 2
     a, b = build(lmw, truncs, precision), build(lmw, truncs, precision)
-    a.macro_steps([600.0, 300.0, 150.0], [1, 2, 4], 3)
-    b.steps([600.0, 300.0, 150.0], [3, 6, 12])
+    a.macro_steps([600.0, 300.0, 150.0], [1, 2, 4], count)
+    b.steps([600.0, 300.0, 150.0], [count, 2 * count, 4 * count])
     for x, y in zip(states(a), states(b)):
         assert np.array_equal(x, y)
-    assert a.diag(9, lmw._lib.DIAG_TIME) == 1800.0 and a.diag(0, lmw._lib.DIAG_TIME) == 1800.0
+    assert a.diag(9, lmw._lib.DIAG_TIME) == 600.0 * count and a.diag(0, lmw._lib.DIAG_TIME) == 600.0 * count
 
 
 @pytest.mark.parametrize("precision", [2, 3])
