@@ -255,9 +255,9 @@ lmw_status lmw_ensemble_steps(lmw_ensemble *e, const double *dts, const int64_t 
 /* replay captured HIP graphs per group (8 RK4 steps, then single steps for the rest; hipGraph path); counts per group */
 lmw_status lmw_ensemble_steps_graph(lmw_ensemble *e, const double *dts, const int64_t *counts);
 /* `count` macro steps: in each, group g advances substeps[g] steps of dts[g] (e.g. 600 s = 1 x 600 s at T42,
- * 2 x 300 s at T85, 4 x 150 s at T170).  One macro step is captured once as a HIP graph whose branches are the
- * groups' streams, so the truncations run concurrently and the host launches one graph per macro step (no reference
- * counterpart) */
+ * 2 x 300 s at T85, 4 x 150 s at T170).  One macro step, and a run of 4 of them, are captured once as HIP graphs
+ * whose branches are the groups' streams, so the truncations run concurrently; the host replays the 4-step graph,
+ * then the single one for the rest (no reference counterpart) */
 lmw_status lmw_ensemble_macro_steps(lmw_ensemble *e, const double *dts, const int *substeps, int64_t count);
 /* persistent macro steps (mixed-precision ensembles of at most 4 truncation groups whose sub-step counts divide the
  * largest): in each stage, one persistent kernel walks a static work list of the inverse Legendre transforms of
