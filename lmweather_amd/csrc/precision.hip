@@ -723,6 +723,25 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
     }
   }
   if constexpr (DMA) wait_vmem();  // the last chunk's repeated request into the LDS buffer
+  // Mixed-precision RK4 epilogue, two row tiles per wave: with NW < nrt row tiles the first nrt - NW waves own a
+  // second tile and would update 8 elements per lane against 4 elsewhere, each element one memory round trip.  The
+  // second tiles' accumulators go through the free operand chunk instead, and wave w updates its own 4 first-tile
+  // elements plus list entries w x .. w x + x - 1 (x = nrt - NW) of the second tiles' elements g (tile NW + g / 4,
+  // register g % 4): nrt elements for every wave.  (Truncations up to 170: nrt <= 6, so x <= 2 and the list takes
+  // at most 16 KB, within every mixed operand chunk.)
+  constexpr bool BAL = PREC != LMW_FP64 && TPW == 2;
+  const int xlist = nrt - NW;
+  float *xl = lds_f;  // [second tile][r][q][lane]
+  if (BAL && mode == ANAL_TEND && xlist > 0) {  // (block-uniform)
+    block_sync();  // every wave is done with the operand chunk
+    if (wave < xlist) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 8; q++) xl[((wave * 4 + r) * 8 + q) * 64 + lane] = accv[TPW - 1][q][r];
+    }
+    block_sync();
+  }
   const int j = lane & 15, bl = mb + j;
   if (bl >= nb) return;
   const int b = b0 + bl, Km = G.N - m + 1;
@@ -733,6 +752,40 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
     return 2 * (16 * (wave + NW * (e >> 2)) + row) + p;
   };
   auto elem_ok = [&](int e) { return wave + NW * (e >> 2) < nrt && elem_k(e) < Km; };
+  if (BAL && mode == ANAL_TEND) {
+    // the first tile's 4 elements from registers, then this wave's list entries from the operand chunk, the state
+    // loads of each element issued before the update of the one before; phi_00 P_0^0 of the stage input requested
+    // with the first element's state
+    auto list_k = [&](int g) { return 2 * (16 * (NW + (g >> 2)) + 4 * (lane >> 4) + (g & 3)) + p; };
+    const int nl = max(xlist, 0), g0 = wave * nl;
+    const double phi00 = mixed ? xs[G.at(b, 2, 0)].x : 0.0;
+    TendIn next = {};
+    if (elem_ok(0)) next = tend_load(G, stage, b, moff_m + elem_k(0), x0, xs, acc, mixed);
+    const double phibar = phi00 * 0.70710678118654752440;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const TendIn cur = next;
+      if (e < 3) {
+        if (elem_ok(e + 1)) next = tend_load(G, stage, b, moff_m + elem_k(e + 1), x0, xs, acc, mixed);
+      } else if (nl > 0 && list_k(g0) < Km) {
+        next = tend_load(G, stage, b, moff_m + list_k(g0), x0, xs, acc, mixed);
+      }
+      if (!elem_ok(e)) continue;
+      const double2 o1 = cx(accv[0][0][e], accv[0][1][e]), o2 = cx(accv[0][2][e], accv[0][3][e]);
+      const double2 o3 = cx(accv[0][4][e], accv[0][5][e]), o4 = cx(accv[0][6][e], accv[0][7][e]);
+      tend_store(G, stage, dt, nu4, b, m + elem_k(e), moff_m + elem_k(e), o1, o2, o3, o4, cur, phibar, x0, xs, acc, mixed);
+    }
+    for (int i = 0; i < nl; i++) {
+      const int g = g0 + i, k = list_k(g);
+      const TendIn cur = next;
+      if (i + 1 < nl && list_k(g + 1) < Km) next = tend_load(G, stage, b, moff_m + list_k(g + 1), x0, xs, acc, mixed);
+      if (k >= Km) continue;
+      const float *xv = xl + g * 8 * 64 + lane;
+      const double2 o1 = cx(xv[0], xv[64]), o2 = cx(xv[128], xv[192]), o3 = cx(xv[256], xv[320]), o4 = cx(xv[384], xv[448]);
+      tend_store(G, stage, dt, nu4, b, m + k, moff_m + k, o1, o2, o3, o4, cur, phibar, x0, xs, acc, mixed);
+    }
+    return;
+  }
   if (mode == ANAL_TEND) {
     // the RK4 update with the state loads of element e + 1 issued before the arithmetic and stores of element e
     // phi_00 P_0^0 of the stage input, requested with the first element's state
