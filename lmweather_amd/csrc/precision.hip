@@ -728,10 +728,11 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
   // second tiles' accumulators go through the free operand chunk instead, and wave w updates its own 4 first-tile
   // elements plus list entries w x .. w x + x - 1 (x = nrt - NW) of the second tiles' elements g (tile NW + g / 4,
   // register g % 4): nrt elements for every wave.  (Truncations up to 170: nrt <= 6, so x <= 2 and the list takes
-  // at most 16 KB, within every mixed operand chunk.)
-  constexpr bool BAL = PREC != LMW_FP64 && TPW == 2;
+  // at most 16 KB of FP32 or 32 KB of FP64 accumulators, within every operand chunk.)  FP64 too: its 16x16x4 MFMA
+  // interleaves the rows (register r of lane l holds row l / 16 + 4 r), which the list's k follows.
+  constexpr bool BAL = TPW == 2;
   const int xlist = nrt - NW;
-  float *xl = lds_f;  // [second tile][r][q][lane]
+  ST *xl = reinterpret_cast<ST *>(lds_f);  // [second tile][r][q][lane]
   if (BAL && mode == ANAL_TEND && xlist > 0) {  // (block-uniform)
     block_sync();  // every wave is done with the operand chunk
     if (wave < xlist) {
@@ -756,7 +757,10 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
     // the first tile's 4 elements from registers, then this wave's list entries from the operand chunk, the state
     // loads of each element issued before the update of the one before; phi_00 P_0^0 of the stage input requested
     // with the first element's state
-    auto list_k = [&](int g) { return 2 * (16 * (NW + (g >> 2)) + 4 * (lane >> 4) + (g & 3)) + p; };
+    auto list_k = [&](int g) {
+      const int row = PREC == LMW_FP64 ? (lane >> 4) + 4 * (g & 3) : 4 * (lane >> 4) + (g & 3);
+      return 2 * (16 * (NW + (g >> 2)) + row) + p;
+    };
     const int nl = max(xlist, 0), g0 = wave * nl;
     const double phi00 = mixed ? xs[G.at(b, 2, 0)].x : 0.0;
     TendIn next = {};
@@ -780,7 +784,7 @@ __device__ __forceinline__ void anal_mx_block(const Geo &G, const AnalTabs &T, c
       const TendIn cur = next;
       if (i + 1 < nl && list_k(g + 1) < Km) next = tend_load(G, stage, b, moff_m + list_k(g + 1), x0, xs, acc, mixed);
       if (k >= Km) continue;
-      const float *xv = xl + g * 8 * 64 + lane;
+      const ST *xv = xl + g * 8 * 64 + lane;
       const double2 o1 = cx(xv[0], xv[64]), o2 = cx(xv[128], xv[192]), o3 = cx(xv[256], xv[320]), o4 = cx(xv[384], xv[448]);
       tend_store(G, stage, dt, nu4, b, m + k, moff_m + k, o1, o2, o3, o4, cur, phibar, x0, xs, acc, mixed);
     }
