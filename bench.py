@@ -504,9 +504,11 @@ def ensemble_bench(lmw, args) -> dict:
         "t170_member_days_per_s": 32 * days / wall,
         "legendre_kernels": "persistent multi-group" if args.persistent else "per group, groups as graph branches",
         "finite": bool(fin),
-        "tiles": {str(t): {"fp64_mb": lmw.sphere.tile_variant(0, int(tl[0]))[0],
-                           "mixed_synth_rc": lmw.sphere.tile_variant(1, int(tl[2]))[0],
-                           "mixed_anal_kc_nw": list(lmw.sphere.tile_variant(2, int(tl[3])))}
+        # FP64: (members per block, 0) of the FMA kernels or (0, row chunk / waves) of the FP64 MFMA kernels
+        "tiles": {str(t): ({"fp64_synth_mb_rc": list(lmw.sphere.tile_variant(0, int(tl[0]))),
+                            "fp64_anal_mb_nw": list(lmw.sphere.tile_variant(3, int(tl[1])))} if prec == lmw.FP64 else
+                           {"mixed_synth_rc": lmw.sphere.tile_variant(1, int(tl[2]))[0],
+                            "mixed_anal_kc_nw": list(lmw.sphere.tile_variant(2, int(tl[3])))})
                   for (t, _, _), tl in zip(ENS_GROUPS, tiles)},
         "kernel_ms_per_macro_step": {str(t): {"inverse": float(m[0]), "grid": float(m[1]), "forward": float(m[2])}
                                      for (t, _, _), m in zip(ENS_GROUPS, ms)},

@@ -411,6 +411,8 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
   // fluxes and the Bernoulli function are formed with phi'; the forward transform's epilogue adds the mean's exact
   // contribution -phibar delta (the mean's part of the Bernoulli function only reaches n = 0, whose tendency is 0)
   const T om = static_cast<T>(1 - mu[jn] * mu[jn]);  // identical for the southern row: mu(s) = -mu(n)
+  // FP32 rows: E = (u^2 + v^2) times 1 / (2 om), rounded once from FP64, instead of four FP32 divisions per thread
+  const T r2om = static_cast<T>(0.5 / (1 - mu[jn] * mu[jn]));
   const T g = static_cast<T>(G.g);
   C res[10];
   if (k < L) {
@@ -430,7 +432,8 @@ __global__ __launch_bounds__(256) void k_tend_grid(Geo G, const TS *__restrict__
         o[1][e] = v * eta;
         o[2][e] = u * ph;
         o[3][e] = v * ph;
-        o[4][e] = ph + g * (flat ? T(0) : hsv[s][e]) + (u * u + v * v) / (2 * om);  // g (h + hs) + E (sphere.cpp:357)
+        const T ke = u * u + v * v;
+        o[4][e] = ph + g * (flat ? T(0) : hsv[s][e]) + (sizeof(T) == 4 ? ke * r2om : ke / (2 * om));  // g (h + hs) + E (sphere.cpp:357)
       }
 #pragma unroll
       for (int q = 0; q < 5; q++) res[2 * q + s] = C{o[q][0], o[q][1]};
