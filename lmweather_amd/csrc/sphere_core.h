@@ -112,8 +112,9 @@ struct SphereCore {
   hipGraphExec_t graph_many = nullptr;  // kGraphSteps steps
   static constexpr int kGraphSteps = 8;
   double graph_dt = 0;
-  // tile variant indices (defaults: FP64 4 members per block, RC 32, KC 32 with 4 waves: the mixed shapes the
-  // autotuner picks at T42, T85 and T170, and the only forward shape with LDS-buffered records)
+  // tile variant indices (defaults: FP64 4 members per block, or the FP64 MFMA kernels for 32 members and more (set
+  // in the constructor); RC 32, KC 32 with 4 waves: the mixed shapes the autotuner picks at T42, T85 and T170, and
+  // the only forward shape with LDS-buffered records)
   int tile_fp64_synth = 1, tile_fp64_anal = 1, tile_synth = 0, tile_anal = 0;
   bool member_minor = false;  // layout of x0/xs/acc (see Geo)
 

@@ -964,6 +964,14 @@ SphereCore::SphereCore(int truncation, const lmw_sphere_params &p, int members, 
   LMW_LAUNCH_CHECK();
   for (int b = 0; b < B; b++) set_axis_tilt(b, 0.0);
   set_precision(prec);
+  // FP64 Legendre kernels of groups of two or more full 16-member tiles: the FP64 MFMA kernels (row chunk 32, 4
+  // waves), which the autotuner picks at T42, T85 and T170 with 32 to 128 members; smaller groups and single members
+  // keep the FMA kernels with 4 members per block
+  if (B >= 32) {
+    tile_fp64_synth = kFp64Tiles;
+    tile_fp64_anal = kFp64Tiles;
+    build_fp64_mfma_tables();
+  }
   sync();
 }
 
