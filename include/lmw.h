@@ -268,7 +268,9 @@ lmw_status lmw_ensemble_set_persistent(lmw_ensemble *e, int on);
  * (restored afterwards) and cached in $LMW_TUNE_CACHE (default ~/.cache/lmweather_amd/tiles_v1.txt) keyed by
  * architecture, truncation, members and precision; force != 0 re-times.  tiles[4 g + 0..3] = variant indices of
  * the FP64 inverse, FP64 forward, mixed inverse and mixed forward transforms of group g; best_ms[2 g + 0..1] = the
- * chosen inverse and forward transform times (0 when taken from the cache).  No reference counterpart. */
+ * chosen inverse and forward transform times (0 when taken from the cache).  Untuned groups use the mixed row chunk
+ * 32 and latitude chunk 32 with 4 waves, and for FP64 the FP64 MFMA kernels from 32 members on, else the FMA
+ * kernels with 4 members per block.  No reference counterpart. */
 lmw_status lmw_ensemble_autotune(lmw_ensemble *e, int force, int *tiles, double *best_ms);
 lmw_status lmw_sphere_autotune(lmw_sphere *s, int force, int *tiles, double *best_ms);
 /* set the four tile variant indices of group g directly (tests, or a caller's own tuning) */
