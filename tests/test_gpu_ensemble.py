@@ -160,3 +160,17 @@ def test_autotune_keeps_state_and_uses_cache(lmw, tmp_path, monkeypatch):
     d.steps([600.0, 150.0], [2, 8])
     for x, y in zip(states(e2), states(d)):
         assert rel_l2(x[2], y[2]) <= 1e-6
+
+
+@pytest.mark.parametrize("precision,n,tiles", [(0, 32, [3, 3, 0, 0]), (0, 20, [1, 1, 0, 0]), (3, 20, [1, 1, 0, 0])])
+def test_untuned_default_tiles(lmw, precision, n, tiles):
+    """Untuned groups: FP64 groups of 32 members step with the FP64 MFMA kernels (row chunk 32, 4 waves), smaller
+    FP64 groups with the FMA kernels of 4 members per block, mixed groups with row chunk 32 and latitude chunk 32
+    on 4 waves: the default stepping is bit-identical to those variants set explicitly."""
+    truncs = [42] * n
+    a, b = build(lmw, truncs, precision), build(lmw, truncs, precision)
+    b.set_tiles(0, tiles)
+    a.steps([600.0], [3])
+    b.steps([600.0], [3])
+    for x, y in zip(states(a), states(b)):
+        assert np.array_equal(x, y)
